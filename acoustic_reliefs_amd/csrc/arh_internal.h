@@ -1,0 +1,244 @@
+// Internal host-side structures of libarh (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/arh.h"
+
+namespace arh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ARH_HIP(call)                                                                                        \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            throw ::arh::Error(ARH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + \
+                                             ":" + std::to_string(__LINE__));                                \
+    } while (0)
+
+#define ARH_CHECK(cond, code, msg)                              \
+    do {                                                        \
+        if (!(cond)) throw ::arh::Error((code), (msg));          \
+    } while (0)
+
+// Owning device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        reset();
+        p = o.p, bytes = o.bytes;
+        o.p = nullptr, o.bytes = 0;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr, bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        reset();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(ARH_ENOMEM, "hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(p);
+    }
+};
+
+template <class T>
+void upload(DBuf &b, const std::vector<T> &v, hipStream_t s) {
+    b.alloc(v.size() * sizeof(T));
+    if (!v.empty()) ARH_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace arh
+
+namespace arh {
+// Live kernel timing with HIP events on the context's stream.
+enum ProfKind { PROF_ROWS64 = 0, PROF_LR64 = 1, PROF_ROWS32 = 2, PROF_LR32 = 3 };
+struct Profiler {
+    struct Entry {
+        hipEvent_t a, b;
+        int kind;
+        double bytes, flops;
+    };
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<Entry> pending;
+    arh_profile acc{};
+    hipEvent_t get() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            ARH_HIP(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    arh_kernel_prof &slot(int kind) {
+        switch (kind) {
+        case PROF_ROWS64: return acc.rows_fp64;
+        case PROF_LR64: return acc.lowrank_fp64;
+        case PROF_ROWS32: return acc.rows_fp32;
+        default: return acc.lowrank_fp32;
+        }
+    }
+    // account finished launches (all of them when wait)
+    void flush(bool wait) {
+        size_t keep = 0;
+        for (size_t i = 0; i < pending.size(); i++) {
+            Entry &e = pending[i];
+            bool done = wait ? (hipEventSynchronize(e.b) == hipSuccess) : (hipEventQuery(e.b) == hipSuccess);
+            if (!done) {
+                pending[keep++] = e;
+                continue;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e.a, e.b);
+            arh_kernel_prof &s = slot(e.kind);
+            s.launches++, s.ms += ms, s.bytes += e.bytes, s.flops += e.flops;
+            pool.push_back(e.a), pool.push_back(e.b);
+        }
+        pending.resize(keep);
+    }
+    ~Profiler() {
+        for (auto &e : pending) (void)hipEventDestroy(e.a), (void)hipEventDestroy(e.b);
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace arh
+
+struct arh_ctx_s {
+    arh::Profiler prof;
+    int refs = 1;          // the caller's handle + one per live geometry
+    bool closed = false;   // arh_ctx_destroy called; freed when the last geometry goes
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // reduction / scalar scratch reused by the solver
+    arh::DBuf red;      // partial sums
+    arh::DBuf scal;     // per-rhs scalars
+    arh::DBuf hostpin;  // pinned host staging
+    size_t hostpin_bytes = 0;
+    void *pinned(size_t bytes);
+};
+
+// Dense-leaf assembly task: row leaf, first panel column, column count (cols * m <= 256).
+struct DenseTask {
+    int leaf, c0, nc, pad;
+};
+
+// Low-rank block descriptor (device). Offsets are in complex elements.
+struct LRBlock {
+    int rs, re, cs, ce;
+    int lim;        // min(max_rank, m-1, n-1)
+    int rank;       // stored rank (crosses with u, v != 0); filled by the ACA
+    int rank_ref;   // rank as the reference counts it (aca.cpp:80)
+    int l0;         // first row leaf
+    int64_t us_off, vs_off;  // ACA scratch (U: m x lim col-major, V: lim x n row-major)
+    int64_t u_off, v_off;    // packed storage (U leaf-segmented, V row-major)
+    int64_t t_off;           // offset of the block's r intermediate values
+    int64_t flag_off;        // used-row flags (m bytes)
+};
+
+struct arh_geom_s {
+    int refs = 1;          // the caller's handle + one per live H-matrix
+    bool closed = false;
+    arh_ctx *ctx = nullptr;
+    int V = 0, N = 0;
+    arh::DBuf P, E, tri;  // Ps (V x 3), Es (N x 3), Tri records
+    // partition (host copies)
+    std::vector<int> dense;    // nd x 4
+    std::vector<int> lowrank;  // na x 4
+    int nd = 0, na = 0;
+    // leaves
+    int nleaf = 0;
+    std::vector<int> leaf_start;  // nleaf + 1
+    std::vector<int> leaf_of;     // N
+    arh::DBuf d_leaf_start;
+    // dense panels: per row leaf, the row's dense blocks side by side (m x ncol column-major)
+    std::vector<int> pcol_ptr;        // nleaf + 1: first panel column of each row leaf
+    std::vector<int64_t> panel_off;   // nleaf + 1: complex offset of each panel
+    std::vector<int> pcol_idx;        // global column (element) of each panel column
+    std::vector<int> dblk_leafr, dblk_leafc, dblk_col;  // per dense block: row leaf, col leaf, panel col
+    arh::DBuf d_pcol_ptr, d_panel_off, d_pcol_idx;
+    int64_t dense_entries = 0;
+    // transposed dense: per col leaf, list of dense blocks
+    std::vector<int> dT_ptr, dT_blk;
+    arh::DBuf d_dT_ptr, d_dT_blk, d_dblk_info;  // dblk_info: (row leaf, panel col) per block
+    // dense assembly tasks
+    std::vector<DenseTask> dtasks;
+    arh::DBuf d_dtasks;
+    // low-rank: per row leaf / per col leaf lists of blocks
+    std::vector<int> lrR_ptr, lrR_blk, lrC_ptr, lrC_blk;
+    arh::DBuf d_lrR_ptr, d_lrR_blk, d_lrC_ptr, d_lrC_blk;
+    // ACA scratch sized for the largest max_rank seen
+    int scratch_rank = -1;
+    arh::DBuf aca_U, aca_V, aca_flags;
+    int64_t aca_u_total = 0, aca_v_total = 0, aca_f_total = 0;
+    std::vector<int64_t> us_off, vs_off, flag_off;
+    std::vector<int> lim;
+};
+
+struct arh_hmat_s {
+    arh_geom *g = nullptr;
+    double k = 0;
+    int storage = ARH_FP64;
+    int max_rank = 0;
+    double aca_tol = 0;
+    arh::DBuf dense, dense32;  // panels (c128 / c64)
+    arh::DBuf blocks;          // LRBlock[na]
+    arh::DBuf U, V, U32, V32;  // packed factors
+    std::vector<LRBlock> hblocks;
+    int64_t u_total = 0, v_total = 0, t_total = 0;
+    std::vector<int> active_lr;  // lr blocks with rank > 0
+    arh::DBuf d_active_lr;
+    arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
+    arh_hmat_info info{};
+};
+
+namespace arh {
+// geometry.hip
+void geom_build_tri(arh_geom *g);
+// assemble.hip
+void assemble_dense(arh_hmat *h);
+void assemble_aca(arh_hmat *h);
+void make_fp32_copy(arh_hmat *h);
+// matvec.hip
+void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+// rhs / receivers / diffusion (receivers.hip)
+void launch_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs_dev, double2 *b);
+void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int transpose, int nrhs, const double2 *x,
+                      double2 *y);
+void launch_scale(hipStream_t s, double2 *a, int64_t n, double f);
+void launch_diffusion(hipStream_t s, int LL, int nrhs, const double2 *y, double *d, double *u, double2 *vf);
+// solver.hip
+void solve(arh_hmat *h, int transpose, int nrhs, const double2 *b, double2 *x, const arh_solve_opts &o,
+           arh_solve_stats *st);
+// grad.hip
+void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const int *diff, int HH, int nrhs,
+                       const double2 *x, const double2 *lam, const double2 *vf, int src_kind, const double *src_dev,
+                       double *A, double *B, double *C);
+}  // namespace arh

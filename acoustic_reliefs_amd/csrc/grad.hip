@@ -1,0 +1,265 @@
+// Shape gradient of the diffusion coefficient w.r.t. vertex heights (y coordinates).
+//
+// Restates compute_der_A/B/C (adjoint.cpp:244-386) restructured per "touched" element i (an
+// element with at least one differentiable vertex) so each derivative is evaluated once and
+// shared by the three corners and by every rhs of a batch:
+//   A_(i,c) = sum_rhs Re( x_i  sum_l  vf_l  dQ_li / dq_(i,c) )                 (adjoint.cpp:268-277)
+//   B_(i,c) = sum_rhs Re( lam_i sum_e x_e dM_ie / dp_i ) + Re( x_i sum_e lam_e dM_ei / dq_(i,c) )   (:314-339)
+//   C_(i,c) = -sum_rhs Re( lam_i db_i / dp_i )                                  (:373)
+// and gradient_h = sum_{(i,c) : E[i][c] = h} A - B + C (element order, then corner order, as the
+// reference's elements[h] lists, acoustics3d.cpp:83-98).
+// Derivatives (derived from the kernel, not transcribed from derivatives.cpp): with d = p - q,
+// r = |d|, s = -(d . n), F = e^{-ikr}(-ikr-1)/(4 pi r^3), F' = e^{-ikr}(-k^2/r^2 + 3(1+ikr)/r^4)/(4 pi):
+//   collocation centroid moves (dp/dh = e_y/3):   dK = (F' d_y s / r - F n_y) / 3
+//   corner c of the integration triangle moves (dq/dh = beta_c e_y, beta = (1-xi-eta, xi, eta)):
+//                                                 dK = -F' beta_c d_y s / r + F (beta_c n_y - d . dn_c)
+//   dn_a = e_y x (b - c), dn_b = e_y x (c - a), dn_c = (b - a) x e_y.
+#include <algorithm>
+#include <vector>
+
+#include "arh_internal.h"
+#include "device_common.h"
+
+namespace arh {
+
+constexpr int kGT = 128;  // threads per workgroup (one touched element each)
+constexpr int kRG = 4;    // rhs per pass
+
+__device__ __forceinline__ void FdF(double k, double r, double2 &F, double2 &dF) {
+    double sn, cs;
+    sincos(k * r, &sn, &cs);
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2;
+    const double kr = k * r;
+    const double inv4pi = 1.0 / (4.0 * kPi);
+    F = make_double2((-cs - kr * sn) / r3 * inv4pi, (sn - kr * cs) / r3 * inv4pi);
+    const double A = -k * k / r2 + 3.0 / r4, B = 3.0 * k / r3;
+    dF = make_double2((cs * A + sn * B) * inv4pi, (cs * B - sn * A) * inv4pi);
+}
+
+// dM(p, tri)/dh for the collocation point p = centroid moving in y (sum over the rule, times 1/2).
+__device__ __forceinline__ double2 dM_dp(double k, double px, double py, double pz, const Tri &t) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int g = 0; g < 6; g++) {
+        double w, xi, eta;
+        quad_point(g, w, xi, eta);
+        const double dx = px - (t.ax + xi * t.bx + eta * t.cx);
+        const double dy = py - (t.ay + xi * t.by + eta * t.cy);
+        const double dz = pz - (t.az + xi * t.bz + eta * t.cz);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double s = -(dx * t.nx + dy * t.ny + dz * t.nz);
+        double2 F, dF;
+        FdF(k, r, F, dF);
+        const double a = dy * s / r;
+        acc.x += w * (dF.x * a - F.x * t.ny);
+        acc.y += w * (dF.y * a - F.y * t.ny);
+    }
+    return make_double2(acc.x * (0.5 / 3.0), acc.y * (0.5 / 3.0));
+}
+
+// dM(p, tri)/dh for corner c = 0, 1, 2 of the integration triangle moving in y.
+__device__ __forceinline__ void dM_dq3(double k, double px, double py, double pz, const Tri &t, double2 out[3]) {
+    // dn for each corner (only x and z components are nonzero)
+    const double bcx = t.bx - t.cx, bcz = t.bz - t.cz;  // (b - a) - (c - a) = b - c
+    const double dnx[3] = {bcz, t.cz, -t.bz};
+    const double dnz[3] = {-bcx, -t.cx, t.bx};
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int g = 0; g < 6; g++) {
+        double w, xi, eta;
+        quad_point(g, w, xi, eta);
+        const double dx = px - (t.ax + xi * t.bx + eta * t.cx);
+        const double dy = py - (t.ay + xi * t.by + eta * t.cy);
+        const double dz = pz - (t.az + xi * t.bz + eta * t.cz);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double s = -(dx * t.nx + dy * t.ny + dz * t.nz);
+        double2 F, dF;
+        FdF(k, r, F, dF);
+        const double beta[3] = {1.0 - xi - eta, xi, eta};
+        const double a = dy * s / r;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double f1 = -beta[c] * a;
+            const double f2 = beta[c] * t.ny - (dx * dnx[c] + dz * dnz[c]);
+            out[c].x += w * (dF.x * f1 + F.x * f2);
+            out[c].y += w * (dF.y * f1 + F.y * f2);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = make_double2(out[c].x * 0.5, out[c].y * 0.5);
+}
+
+// B partials: grid.x over touched elements, grid.y splits the element loop; writes
+// partB[(split * T + t) * 3 + c] (one pass over rhs j0..j0+kRG-1, accumulated over passes by the caller).
+__global__ __launch_bounds__(kGT) void k_grad_B(const Tri *__restrict__ tri, int N, int T, const int *__restrict__ touched,
+                                                double k, int R, int j0, const double2 *__restrict__ x,
+                                                const double2 *__restrict__ lam, int esplit,
+                                                double *__restrict__ partB) {
+    const int t = blockIdx.x * kGT + threadIdx.x;
+    const int sp = blockIdx.y;
+    if (t >= T) return;
+    const int i = touched[t];
+    const Tri ti = tri[i];
+    const int len = (N + esplit - 1) / esplit;
+    const int e0 = sp * len, e1 = min(N, e0 + len);
+    double2 s1[kRG], s2[kRG][3];
+#pragma unroll
+    for (int q = 0; q < kRG; q++) {
+        s1[q] = make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; c++) s2[q][c] = make_double2(0.0, 0.0);
+    }
+    for (int e = e0; e < e1; e++) {
+        const Tri te = tri[e];
+        const double2 dp = dM_dp(k, ti.gx, ti.gy, ti.gz, te);   // M_ie, collocation i moves
+        double2 dq[3];
+        dM_dq3(k, te.gx, te.gy, te.gz, ti, dq);                  // M_ei, corner c of i moves
+#pragma unroll
+        for (int q = 0; q < kRG; q++) {
+            if (j0 + q >= R) break;
+            const double2 xe = x[(int64_t)(j0 + q) * N + e];
+            const double2 le = lam[(int64_t)(j0 + q) * N + e];
+            cfma(s1[q], xe, dp);
+#pragma unroll
+            for (int c = 0; c < 3; c++) cfma(s2[q][c], le, dq[c]);
+        }
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kRG; q++) {
+        if (j0 + q >= R) break;
+        const double2 li = lam[(int64_t)(j0 + q) * N + i];
+        const double2 xi = x[(int64_t)(j0 + q) * N + i];
+        const double a = cmul(s1[q], li).x;
+#pragma unroll
+        for (int c = 0; c < 3; c++) out[c] += a + cmul(s2[q][c], xi).x;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) partB[((int64_t)sp * T + t) * 3 + c] += out[c];
+}
+
+// A and C per touched element (listeners: LL is small), accumulated over rhs passes.
+__global__ __launch_bounds__(kGT) void k_grad_AC(const Tri *__restrict__ tri, int N, int T,
+                                                 const int *__restrict__ touched, double k, int R,
+                                                 const double2 *__restrict__ x, const double2 *__restrict__ lam,
+                                                 const double2 *__restrict__ vf, const double *__restrict__ L, int LL,
+                                                 int src_kind, const double *__restrict__ src,
+                                                 double *__restrict__ cA, double *__restrict__ cC) {
+    const int t = blockIdx.x * kGT + threadIdx.x;
+    if (t >= T) return;
+    const int i = touched[t];
+    const Tri ti = tri[i];
+    double a[3] = {0.0, 0.0, 0.0}, cc = 0.0;
+    for (int j = 0; j < R; j++) {
+        double2 sa[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+        for (int l = 0; l < LL; l++) {
+            double2 dq[3];
+            dM_dq3(k, L[3 * l], L[3 * l + 1], L[3 * l + 2], ti, dq);
+            const double2 v = vf[(int64_t)j * LL + l];
+#pragma unroll
+            for (int c = 0; c < 3; c++) cfma(sa[c], v, dq[c]);
+        }
+        const double2 xi = x[(int64_t)j * N + i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[c] += cmul(sa[c], xi).x;
+        // db_i / dh
+        double2 db;
+        const double sx = src[3 * j], sy = src[3 * j + 1], sz = src[3 * j + 2];
+        if (src_kind == ARH_RHS_POINT) {
+            const double dx = ti.gx - sx, dy = ti.gy - sy, dz = ti.gz - sz;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double sn, cs;
+            sincos(k * r, &sn, &cs);
+            // e^{-ikr}(-1 - ikr) / (4 pi r^2) * d_y / (3 r)
+            const double f = dy / (3.0 * r) / (4.0 * kPi * r * r);
+            db = make_double2((-cs - k * r * sn) * f, (sn - k * r * cs) * f);
+        } else {
+            double sn, cs;
+            sincos(k * (sx * ti.gx + sy * ti.gy + sz * ti.gz), &sn, &cs);
+            // (i f) e^{-i k d.c} with f = -k d_y / 3
+            const double f = -k * sy / 3.0;
+            db = make_double2(f * sn, f * cs);
+        }
+        cc -= cmul(db, lam[(int64_t)j * N + i]).x;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) cA[t * 3 + c] = a[c], cC[t * 3 + c] = cc;
+}
+
+// gradient_h: sum over (touched element, corner) in list order; B summed over the splits first.
+__global__ void k_grad_scatter(int HH, int T, int esplit, const int *__restrict__ hptr, const int *__restrict__ hlist,
+                               const double *__restrict__ cA, const double *__restrict__ partB,
+                               const double *__restrict__ cC, double *__restrict__ A, double *__restrict__ B,
+                               double *__restrict__ C) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= HH) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int e = hptr[h]; e < hptr[h + 1]; e++) {
+        const int tc = hlist[e];  // t * 3 + corner
+        a += cA[tc];
+        double bb = 0.0;
+        for (int sp = 0; sp < esplit; sp++) bb += partB[(int64_t)sp * T * 3 + tc];
+        b += bb;
+        c += cC[tc];
+    }
+    A[h] = a, B[h] = b, C[h] = c;
+}
+
+void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const int *diff, int HH, int nrhs,
+                       const double2 *x, const double2 *lam, const double2 *vf, int src_kind, const double *src_dev,
+                       double *A, double *B, double *C) {
+    hipStream_t s = g->ctx->stream;
+    const int N = g->N, V = g->V;
+    // slots of differentiable vertices; touched elements; per-h (element, corner) lists
+    std::vector<int> slot(V, -1);
+    for (int h = 0; h < HH; h++) {
+        ARH_CHECK(diff[h] >= 0 && diff[h] < V, ARH_EINVAL, "diff vertex id out of range");
+        slot[diff[h]] = h;  // a repeated id keeps its last slot, as the reference's map assignment does
+    }
+    std::vector<int> Eh(3 * (size_t)N);
+    ARH_HIP(hipMemcpyAsync(Eh.data(), g->E.p, sizeof(int) * 3 * N, hipMemcpyDeviceToHost, s));
+    ARH_HIP(hipStreamSynchronize(s));
+    std::vector<int> touched, tpos(N, -1);
+    std::vector<std::vector<int>> lists(HH);
+    for (int e = 0; e < N; e++)
+        for (int c = 0; c < 3; c++) {
+            const int h = slot[Eh[3 * e + c]];
+            if (h < 0) continue;
+            if (tpos[e] < 0) tpos[e] = (int)touched.size(), touched.push_back(e);
+            lists[h].push_back(tpos[e] * 3 + c);
+        }
+    const int T = (int)touched.size();
+    std::vector<int> hptr(HH + 1, 0), hlist;
+    for (int h = 0; h < HH; h++) {
+        hlist.insert(hlist.end(), lists[h].begin(), lists[h].end());
+        hptr[h + 1] = (int)hlist.size();
+    }
+    if (T == 0) {
+        ARH_HIP(hipMemsetAsync(A, 0, sizeof(double) * HH, s));
+        ARH_HIP(hipMemsetAsync(B, 0, sizeof(double) * HH, s));
+        ARH_HIP(hipMemsetAsync(C, 0, sizeof(double) * HH, s));
+        ARH_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    const int gx = (T + kGT - 1) / kGT;
+    const int esplit = std::max(1, std::min(std::max(1, N / 64), (2048 + gx - 1) / gx));
+    DBuf d_touched, d_hptr, d_hlist, cA, cC, pB;
+    upload(d_touched, touched, s);
+    upload(d_hptr, hptr, s);
+    upload(d_hlist, hlist, s);
+    cA.alloc(sizeof(double) * 3 * T);
+    cC.alloc(sizeof(double) * 3 * T);
+    pB.alloc(sizeof(double) * 3 * (size_t)T * esplit);
+    ARH_HIP(hipMemsetAsync(pB.p, 0, sizeof(double) * 3 * (size_t)T * esplit, s));
+    for (int j0 = 0; j0 < nrhs; j0 += kRG)
+        hipLaunchKernelGGL(k_grad_B, dim3(gx, esplit), dim3(kGT), 0, s, g->tri.as<Tri>(), N, T, d_touched.as<int>(), k,
+                           nrhs, j0, x, lam, esplit, pB.as<double>());
+    hipLaunchKernelGGL(k_grad_AC, dim3(gx), dim3(kGT), 0, s, g->tri.as<Tri>(), N, T, d_touched.as<int>(), k, nrhs, x,
+                       lam, vf, L_dev, LL, src_kind, src_dev, cA.as<double>(), cC.as<double>());
+    hipLaunchKernelGGL(k_grad_scatter, dim3((HH + 255) / 256), dim3(256), 0, s, HH, T, esplit, d_hptr.as<int>(),
+                       d_hlist.as<int>(), cA.as<double>(), pB.as<double>(), cC.as<double>(), A, B, C);
+    ARH_HIP(hipGetLastError());
+    ARH_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace arh

@@ -1,0 +1,295 @@
+// H-matrix products y = (M - I/2) x and y = (M - I/2)^T x for nrhs vectors (hmatrix.h:84-140).
+//
+// Forward, two launches, no atomics (every output row has one owner, so the result does not
+// depend on the batch it is computed in):
+//   1. per low-rank block b:   t_b = V_b x[cs:ce]                      (r_b values per rhs)
+//   2. per row leaf l:         y[l] = -x[l]/2 + P_l x[cols(P_l)] + sum_{b covers l} U_b[l,:] t_b
+//      P_l is the leaf's dense panel (its dense blocks side by side, column-major), U_b[l,:] the
+//      leaf's contiguous segment of U_b. Lanes cover (row, column) pairs of a column-major tile,
+//      so every load of a wavefront is one contiguous run.
+// Transposed (set_transposed, hmatrix.h:95-115):
+//   1. per low-rank block b:   s_b = U_b^T x[rs:re]
+//   2. per column leaf l:      y[l] = -x[l]/2 + sum_{dense (l', l)} D^T x[l'] + sum_{b covers l} V_b[:, l]^T s_b
+#include <algorithm>
+
+#include "arh_internal.h"
+#include "device_common.h"
+
+namespace arh {
+
+constexpr int kMvThreads = 256;
+constexpr int kMvWaves = kMvThreads / ARH_WAVE;
+constexpr int RC = 4;  // right-hand sides per pass
+
+template <class S>
+struct CT;
+template <>
+struct CT<double2> {
+    __device__ static double2 ld(const double2 *p) { return *p; }
+};
+template <>
+struct CT<float2> {
+    __device__ static double2 ld(const float2 *p) {
+        float2 v = *p;
+        return make_double2(v.x, v.y);
+    }
+};
+
+// ---------------------------------------------------------------- phase 1 (forward): t = V x
+template <class S>
+__global__ __launch_bounds__(kMvThreads) void k_lr_t(const LRBlock *__restrict__ blocks, const int *__restrict__ active,
+                                                     const S *__restrict__ V, const double2 *__restrict__ x, int N,
+                                                     int R, double2 *__restrict__ t) {
+    const LRBlock B = blocks[active[blockIdx.x]];
+    const int n = B.ce - B.cs, r = B.rank;
+    const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
+    for (int p = w; p < r; p += kMvWaves) {
+        const S *vr = V + B.v_off + (int64_t)p * n;
+        for (int j0 = 0; j0 < R; j0 += RC) {
+            double2 acc[RC];
+#pragma unroll
+            for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
+            for (int c = lane; c < n; c += ARH_WAVE) {
+                double2 v = CT<S>::ld(vr + c);
+#pragma unroll
+                for (int q = 0; q < RC; q++)
+                    if (j0 + q < R) cfma(acc[q], v, x[(int64_t)(j0 + q) * N + B.cs + c]);
+            }
+#pragma unroll
+            for (int q = 0; q < RC; q++) {
+                double2 s = wave_sum2(acc[q]);
+                if (lane == 0 && j0 + q < R) t[(B.t_off + p) * R + j0 + q] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- phase 1 (transposed): s = U^T x
+template <class S>
+__global__ __launch_bounds__(kMvThreads) void k_lr_s(const LRBlock *__restrict__ blocks, const int *__restrict__ active,
+                                                     const int *__restrict__ leaf_start, const S *__restrict__ U,
+                                                     const double2 *__restrict__ x, int N, int R,
+                                                     double2 *__restrict__ t) {
+    const LRBlock B = blocks[active[blockIdx.x]];
+    const int r = B.rank;
+    const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
+    for (int p = w; p < r; p += kMvWaves) {
+        for (int j0 = 0; j0 < R; j0 += RC) {
+            double2 acc[RC];
+#pragma unroll
+            for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
+            for (int l = B.l0; leaf_start[l] < B.re; l++) {
+                const int ls = leaf_start[l], ms = leaf_start[l + 1] - ls;
+                const S *col = U + B.u_off + (int64_t)(ls - B.rs) * r + (int64_t)p * ms;
+                for (int i = lane; i < ms; i += ARH_WAVE) {
+                    double2 u = CT<S>::ld(col + i);
+#pragma unroll
+                    for (int q = 0; q < RC; q++)
+                        if (j0 + q < R) cfma(acc[q], u, x[(int64_t)(j0 + q) * N + ls + i]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RC; q++) {
+                double2 s = wave_sum2(acc[q]);
+                if (lane == 0 && j0 + q < R) t[(B.t_off + p) * R + j0 + q] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- phase 2 (forward): one workgroup per row leaf
+template <class S>
+__global__ __launch_bounds__(kMvThreads) void k_rows(const int *__restrict__ leaf_start, const int *__restrict__ pcol_ptr,
+                                                     const int64_t *__restrict__ panel_off,
+                                                     const int *__restrict__ pcol_idx, const S *__restrict__ panel,
+                                                     const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
+                                                     const LRBlock *__restrict__ blocks, const S *__restrict__ U,
+                                                     const double2 *__restrict__ t, const double2 *__restrict__ x,
+                                                     int N, int R, double2 *__restrict__ y) {
+    __shared__ double2 red[kMvWaves * ARH_WAVE * RC];
+    const int l = blockIdx.x;
+    const int rs = leaf_start[l], m = leaf_start[l + 1] - rs;
+    const int ncol = pcol_ptr[l + 1] - pcol_ptr[l];
+    const S *P = panel + panel_off[l];
+    const int *cidx = pcol_idx + pcol_ptr[l];
+    const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
+    const int b0 = lr_ptr[l], b1 = lr_ptr[l + 1];
+    for (int rc = 0; rc < m; rc += ARH_WAVE) {
+        const int mm = min(ARH_WAVE, m - rc);
+        const int cpl = ARH_WAVE / mm;  // columns per wavefront pass
+        const int cc = lane / mm, i = lane - cc * mm;
+        const bool on = cc < cpl;
+        const int step = kMvWaves * cpl;
+        for (int j0 = 0; j0 < R; j0 += RC) {
+            double2 acc[RC];
+#pragma unroll
+            for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
+            if (on) {
+                for (int c = w * cpl + cc; c < ncol; c += step) {
+                    double2 a = CT<S>::ld(P + (int64_t)c * m + rc + i);
+                    const int col = cidx[c];
+#pragma unroll
+                    for (int q = 0; q < RC; q++)
+                        if (j0 + q < R) cfma(acc[q], a, x[(int64_t)(j0 + q) * N + col]);
+                }
+                for (int e = b0; e < b1; e++) {
+                    const LRBlock &B = blocks[lr_blk[e]];
+                    const int r = B.rank;
+                    const S *seg = U + B.u_off + (int64_t)(rs - B.rs) * r;
+                    for (int p = w * cpl + cc; p < r; p += step) {
+                        double2 a = CT<S>::ld(seg + (int64_t)p * m + rc + i);
+#pragma unroll
+                        for (int q = 0; q < RC; q++)
+                            if (j0 + q < R) cfma(acc[q], a, t[(B.t_off + p) * R + j0 + q]);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RC; q++) red[(w * ARH_WAVE + lane) * RC + q] = acc[q];
+            __syncthreads();
+            if (threadIdx.x < mm * RC) {
+                const int ii = threadIdx.x % mm, q = threadIdx.x / mm;
+                if (j0 + q < R) {
+                    const int64_t yo = (int64_t)(j0 + q) * N + rs + rc + ii;
+                    double2 s = make_double2(-0.5 * x[yo].x, -0.5 * x[yo].y);
+                    for (int ww = 0; ww < kMvWaves; ww++)
+                        for (int c2 = 0; c2 < cpl; c2++) {
+                            double2 v = red[(ww * ARH_WAVE + c2 * mm + ii) * RC + q];
+                            s.x += v.x, s.y += v.y;
+                        }
+                    y[yo] = s;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- phase 2 (transposed): one workgroup per column leaf
+template <class S>
+__global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ leaf_start, const int *__restrict__ dT_ptr,
+                                                     const int *__restrict__ dT_blk, const int *__restrict__ dblk_info,
+                                                     const int64_t *__restrict__ panel_off, const S *__restrict__ panel,
+                                                     const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
+                                                     const LRBlock *__restrict__ blocks, const S *__restrict__ V,
+                                                     const double2 *__restrict__ t, const double2 *__restrict__ x,
+                                                     int N, int R, double2 *__restrict__ y) {
+    __shared__ double2 red[kMvWaves * ARH_WAVE * RC];
+    const int l = blockIdx.x;
+    const int cs = leaf_start[l], n = leaf_start[l + 1] - cs;
+    const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
+    const int d0 = dT_ptr[l], d1 = dT_ptr[l + 1];
+    const int b0 = lr_ptr[l], b1 = lr_ptr[l + 1];
+    for (int c0 = 0; c0 < n; c0 += ARH_WAVE) {
+        const int c = c0 + lane;
+        const bool on = c < n;
+        for (int j0 = 0; j0 < R; j0 += RC) {
+            double2 acc[RC];
+#pragma unroll
+            for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
+            if (on) {
+                for (int e = d0 + w; e < d1; e += kMvWaves) {
+                    const int b = dT_blk[e];
+                    const int lr = dblk_info[2 * b], pc = dblk_info[2 * b + 1];
+                    const int rs = leaf_start[lr], m = leaf_start[lr + 1] - rs;
+                    const S *D = panel + panel_off[lr] + (int64_t)pc * m + (int64_t)c * m;
+                    for (int i = 0; i < m; i++) {
+                        double2 a = CT<S>::ld(D + i);
+#pragma unroll
+                        for (int q = 0; q < RC; q++)
+                            if (j0 + q < R) cfma(acc[q], a, x[(int64_t)(j0 + q) * N + rs + i]);
+                    }
+                }
+                for (int e = b0; e < b1; e++) {
+                    const LRBlock &B = blocks[lr_blk[e]];
+                    const int r = B.rank, nb = B.ce - B.cs;
+                    const S *vc = V + B.v_off + (cs - B.cs) + c;
+                    for (int p = w; p < r; p += kMvWaves) {
+                        double2 a = CT<S>::ld(vc + (int64_t)p * nb);
+#pragma unroll
+                        for (int q = 0; q < RC; q++)
+                            if (j0 + q < R) cfma(acc[q], a, t[(B.t_off + p) * R + j0 + q]);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RC; q++) red[(w * ARH_WAVE + lane) * RC + q] = acc[q];
+            __syncthreads();
+            const int nn = min(ARH_WAVE, n - c0);
+            if (threadIdx.x < nn * RC) {
+                const int cc = threadIdx.x % nn, q = threadIdx.x / nn;
+                if (j0 + q < R) {
+                    const int64_t yo = (int64_t)(j0 + q) * N + cs + c0 + cc;
+                    double2 s = make_double2(-0.5 * x[yo].x, -0.5 * x[yo].y);
+                    for (int ww = 0; ww < kMvWaves; ww++) {
+                        double2 v = red[(ww * ARH_WAVE + cc) * RC + q];
+                        s.x += v.x, s.y += v.y;
+                    }
+                    y[yo] = s;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <class S>
+static void apply_t(arh_hmat *h, int transpose, int R, const S *panel, const S *U, const S *V, const double2 *x,
+                    double2 *y, hipStream_t s) {
+    arh_geom *g = h->g;
+    Profiler &pf = g->ctx->prof;
+    const int N = g->N;
+    h->tbuf.alloc(sizeof(double2) * std::max<int64_t>(1, h->t_total * R));
+    double2 *t = h->tbuf.as<double2>();
+    const int nact = (int)h->active_lr.size();
+    const double es = (double)sizeof(S), vs = 16.0 * R;
+    const int64_t lr_entries = transpose ? h->u_total : h->v_total;   // phase-1 factor
+    const int64_t rw_entries = g->dense_entries + (transpose ? h->v_total : h->u_total);
+    const bool fp32 = sizeof(S) == sizeof(float2);
+    auto timed = [&](int kind, double bytes, double flops, auto &&launch) {
+        if (!pf.on) {
+            launch();
+            return;
+        }
+        Profiler::Entry e{pf.get(), pf.get(), kind, bytes, flops};
+        ARH_HIP(hipEventRecord(e.a, s));
+        launch();
+        ARH_HIP(hipEventRecord(e.b, s));
+        pf.pending.push_back(e);
+    };
+    if (nact)
+        timed(fp32 ? PROF_LR32 : PROF_LR64, es * lr_entries + vs * (N + h->t_total), 8.0 * R * lr_entries, [&] {
+            if (!transpose)
+                hipLaunchKernelGGL(k_lr_t<S>, dim3(nact), dim3(kMvThreads), 0, s, h->blocks.as<LRBlock>(),
+                                   h->d_active_lr.as<int>(), V, x, N, R, t);
+            else
+                hipLaunchKernelGGL(k_lr_s<S>, dim3(nact), dim3(kMvThreads), 0, s, h->blocks.as<LRBlock>(),
+                                   h->d_active_lr.as<int>(), g->d_leaf_start.as<int>(), U, x, N, R, t);
+        });
+    timed(fp32 ? PROF_ROWS32 : PROF_ROWS64, es * rw_entries + vs * (2.0 * N + h->t_total), 8.0 * R * rw_entries, [&] {
+        if (!transpose)
+            hipLaunchKernelGGL(k_rows<S>, dim3(g->nleaf), dim3(kMvThreads), 0, s, g->d_leaf_start.as<int>(),
+                               g->d_pcol_ptr.as<int>(), g->d_panel_off.as<int64_t>(), g->d_pcol_idx.as<int>(), panel,
+                               g->d_lrR_ptr.as<int>(), g->d_lrR_blk.as<int>(), h->blocks.as<LRBlock>(), U, t, x, N, R,
+                               y);
+        else
+            hipLaunchKernelGGL(k_cols<S>, dim3(g->nleaf), dim3(kMvThreads), 0, s, g->d_leaf_start.as<int>(),
+                               g->d_dT_ptr.as<int>(), g->d_dT_blk.as<int>(), g->d_dblk_info.as<int>(),
+                               g->d_panel_off.as<int64_t>(), panel, g->d_lrC_ptr.as<int>(), g->d_lrC_blk.as<int>(),
+                               h->blocks.as<LRBlock>(), V, t, x, N, R, y);
+    });
+    ARH_HIP(hipGetLastError());
+}
+
+void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s) {
+    ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
+    if (precision == ARH_MIXED) {
+        ARH_CHECK(h->storage == ARH_MIXED, ARH_ESTATE, "FP32 product requested but the H-matrix has no FP32 copy");
+        apply_t<float2>(h, transpose, nrhs, h->dense32.as<float2>(), h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
+    } else {
+        apply_t<double2>(h, transpose, nrhs, h->dense.as<double2>(), h->U.as<double2>(), h->V.as<double2>(), x, y, s);
+    }
+}
+
+}  // namespace arh

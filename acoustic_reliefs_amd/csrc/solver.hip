@@ -1,0 +1,523 @@
+// Batched BiCGSTAB on the H-matrix, one independent iteration per right-hand side.
+//
+// Per rhs it follows Eigen 3.4's BiCGSTAB (the solver the reference instantiates as
+// Eigen::BiCGSTAB<HMatrix, Eigen::IdentityPreconditioner>, adjoint.cpp:92-95):
+//   x = 0, r = b, r0 = r, rho = alpha = w = 1, v = p = 0
+//   while ||r||^2 > tol^2 ||b||^2 and i < maxit:
+//     rho_old = rho; rho = r0^H r
+//     if |rho| < eps^2 ||r0||^2: r = b - A x; r0 = r; rho = ||r0||^2 (first restart resets i)
+//     beta = (rho / rho_old)(alpha / w); p = r + beta (p - w v); v = A p
+//     alpha = rho / (r0^H v); s = r - alpha v; t = A s
+//     w = t^H s / ||t||^2 (0 if ||t|| = 0); x += alpha p + w s; r = s - w t; i++
+// The reductions are two-stage with a fixed split, so every rhs gets the same bits whatever
+// the batch. One small device->host copy per iteration carries ||r||^2 and the next rho.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "arh_internal.h"
+#include "device_common.h"
+
+namespace arh {
+
+struct RhsState {
+    double2 rho;      // rho of the previous iteration (Eigen's rho_old at the next step)
+    double2 rho_new;  // r0^H r for the coming iteration
+    double2 alpha, w;
+    double r0sq, rhs_sq, rn, pad;
+    int active, restart, pad0, pad1;
+};
+
+constexpr int kVT = 256;      // threads per block of vector kernels
+constexpr int kMaxNB = 128;   // blocks per rhs of a reduction
+
+__device__ __forceinline__ void block_reduce4(double v[4], double *sh) {
+    const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++) sh[w * 4 + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) {
+            double s = sh[k];
+            for (int q = 1; q < kVT / ARH_WAVE; q++) s += sh[q * 4 + k];
+            v[k] = s;
+        }
+}
+
+// fixed-order sum of NB partial quadruples
+__device__ __forceinline__ void sum_parts(const double *part, int NB, double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) out[k] += part[b * 4 + k];
+}
+
+struct Chunk {
+    int lo, hi;
+};
+__device__ __forceinline__ Chunk chunk_of(int N, int NB) {
+    int len = (N + NB - 1) / NB;
+    int lo = blockIdx.x * len;
+    return Chunk{lo, min(N, lo + len)};
+}
+
+// partial sums of conj(a) b (complex) and |c|^2 over the block's chunk
+__global__ __launch_bounds__(kVT) void k_dot(const RhsState *__restrict__ S, int N, int NB,
+                                             const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                             const double2 *__restrict__ c, double *__restrict__ part) {
+    __shared__ double sh[4 * kVT / ARH_WAVE];
+    const int j = blockIdx.y;
+    if (!S[j].active) return;
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    double v[4] = {0, 0, 0, 0};
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) {
+        double2 ai = a[o + i], bi = b[o + i];
+        v[0] += ai.x * bi.x + ai.y * bi.y;
+        v[1] += ai.x * bi.y - ai.y * bi.x;
+        if (c) {
+            double2 ci = c[o + i];
+            v[2] += ci.x * ci.x + ci.y * ci.y;
+        }
+    }
+    block_reduce4(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
+}
+
+// p = r + beta (p - w v), beta = (rho_new / rho)(alpha / w)
+__global__ __launch_bounds__(kVT) void k_p_update(const RhsState *__restrict__ S, int N, int NB,
+                                                  const double2 *__restrict__ r, double2 *__restrict__ p,
+                                                  const double2 *__restrict__ v) {
+    const int j = blockIdx.y;
+    const RhsState st = S[j];
+    if (!st.active) return;
+    const double2 beta = cmul(cdiv(st.rho_new, st.rho), cdiv(st.alpha, st.w));
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) {
+        double2 pi = p[o + i], vi = v[o + i], ri = r[o + i];
+        double2 q = csub(pi, cmul(st.w, vi));
+        p[o + i] = cadd(ri, cmul(beta, q));
+    }
+}
+
+// s = r - alpha v, alpha = rho_new / (r0^H v) from the partials
+__global__ __launch_bounds__(kVT) void k_s_update(const RhsState *__restrict__ S, int N, int NB,
+                                                  const double *__restrict__ part_r0v, const double2 *__restrict__ r,
+                                                  const double2 *__restrict__ v, double2 *__restrict__ s) {
+    const int j = blockIdx.y;
+    const RhsState st = S[j];
+    if (!st.active) return;
+    double d[4];
+    sum_parts(part_r0v + (int64_t)j * NB * 4, NB, d);
+    const double2 alpha = cdiv(st.rho_new, make_double2(d[0], d[1]));
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) s[o + i] = csub(r[o + i], cmul(alpha, v[o + i]));
+}
+
+// x += alpha p + w s; r = s - w t; partials of r0^H r and |r|^2
+__global__ __launch_bounds__(kVT) void k_xr_update(const RhsState *__restrict__ S, int N, int NB,
+                                                   const double *__restrict__ part_r0v,
+                                                   const double *__restrict__ part_ts, const double2 *__restrict__ p,
+                                                   const double2 *__restrict__ s, const double2 *__restrict__ t,
+                                                   const double2 *__restrict__ r0, double2 *__restrict__ x,
+                                                   double2 *__restrict__ r, double *__restrict__ part_rr) {
+    __shared__ double sh[4 * kVT / ARH_WAVE];
+    const int j = blockIdx.y;
+    const RhsState st = S[j];
+    if (!st.active) return;
+    double d[4], e[4];
+    sum_parts(part_r0v + (int64_t)j * NB * 4, NB, d);
+    sum_parts(part_ts + (int64_t)j * NB * 4, NB, e);
+    const double2 alpha = cdiv(st.rho_new, make_double2(d[0], d[1]));
+    const double tt = e[2];
+    const double2 w = tt > 0.0 ? make_double2(e[0] / tt, e[1] / tt) : make_double2(0.0, 0.0);
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    double v[4] = {0, 0, 0, 0};
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) {
+        double2 si = s[o + i], ti = t[o + i];
+        double2 xi = x[o + i];
+        xi = cadd(xi, cadd(cmul(alpha, p[o + i]), cmul(w, si)));
+        x[o + i] = xi;
+        double2 ri = csub(si, cmul(w, ti));
+        r[o + i] = ri;
+        double2 qi = r0[o + i];
+        v[0] += qi.x * ri.x + qi.y * ri.y;
+        v[1] += qi.x * ri.y - qi.y * ri.x;
+        v[2] += ri.x * ri.x + ri.y * ri.y;
+    }
+    block_reduce4(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part_rr[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
+}
+
+// end of an iteration: commit alpha, w, rho; rho_new = r0^H r, rn = |r|^2
+__global__ void k_finish(RhsState *__restrict__ S, int R, int NB, const double *__restrict__ part_r0v,
+                         const double *__restrict__ part_ts, const double *__restrict__ part_rr) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= R || !S[j].active) return;
+    RhsState st = S[j];
+    double d[4], e[4], f[4];
+    sum_parts(part_r0v + (int64_t)j * NB * 4, NB, d);
+    sum_parts(part_ts + (int64_t)j * NB * 4, NB, e);
+    sum_parts(part_rr + (int64_t)j * NB * 4, NB, f);
+    st.alpha = cdiv(st.rho_new, make_double2(d[0], d[1]));
+    st.w = e[2] > 0.0 ? make_double2(e[0] / e[2], e[1] / e[2]) : make_double2(0.0, 0.0);
+    st.rho = st.rho_new;
+    st.rho_new = make_double2(f[0], f[1]);
+    st.rn = f[2];
+    S[j] = st;
+}
+
+// (re)start of rhs j: r = b - Ax (Ax given, or zero), r0 = r, partials of |r|^2 (and b^2 on init)
+__global__ __launch_bounds__(kVT) void k_restart(const RhsState *__restrict__ S, int N, int NB,
+                                                 const double2 *__restrict__ b, const double2 *__restrict__ Ax,
+                                                 double2 *__restrict__ r, double2 *__restrict__ r0,
+                                                 double *__restrict__ part) {
+    __shared__ double sh[4 * kVT / ARH_WAVE];
+    const int j = blockIdx.y;
+    if (!S[j].restart) return;
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    double v[4] = {0, 0, 0, 0};
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) {
+        double2 bi = b[o + i];
+        double2 ri = Ax ? csub(bi, Ax[o + i]) : bi;
+        r[o + i] = ri;
+        r0[o + i] = ri;
+        v[0] += ri.x * ri.x + ri.y * ri.y;
+        v[1] += bi.x * bi.x + bi.y * bi.y;
+    }
+    block_reduce4(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
+}
+
+// full restart: rho = r0sq = |r|^2; fresh: also alpha = w = rho_old = 1, rhs_sq, and the caller zeroes p, v
+__global__ void k_restart_finish(RhsState *__restrict__ S, int R, int NB, const double *__restrict__ part,
+                                 int fresh) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= R || !S[j].restart) return;
+    RhsState st = S[j];
+    double f[4];
+    sum_parts(part + (int64_t)j * NB * 4, NB, f);
+    st.r0sq = f[0];
+    st.rho_new = make_double2(f[0], 0.0);
+    st.rn = f[0];
+    if (fresh) {
+        st.rhs_sq = f[1];
+        st.rho = make_double2(1.0, 0.0);
+        st.alpha = make_double2(1.0, 0.0);
+        st.w = make_double2(1.0, 0.0);
+    }
+    st.restart = 0;
+    S[j] = st;
+}
+
+// masked zeroing of columns (p, v on fresh starts; x on init)
+__global__ void k_zero_cols(const RhsState *__restrict__ S, int N, double2 *__restrict__ a, int use_restart_mask) {
+    const int j = blockIdx.y;
+    if (use_restart_mask && !S[j].restart) return;
+    const int64_t o = (int64_t)j * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        a[o + i] = make_double2(0.0, 0.0);
+}
+
+// x += d (refinement), masked by `active`
+__global__ void k_axpy_cols(const int *__restrict__ mask, int N, double2 *__restrict__ x,
+                            const double2 *__restrict__ d) {
+    const int j = blockIdx.y;
+    if (!mask[j]) return;
+    const int64_t o = (int64_t)j * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) x[o + i] = cadd(x[o + i], d[o + i]);
+}
+
+// r = b - Ax, per rhs partials of |r|^2 and |b|^2 (true residual)
+__global__ __launch_bounds__(kVT) void k_resid(int N, int NB, const double2 *__restrict__ b,
+                                               const double2 *__restrict__ Ax, double2 *__restrict__ r,
+                                               double *__restrict__ part) {
+    __shared__ double sh[4 * kVT / ARH_WAVE];
+    const int j = blockIdx.y;
+    const Chunk ch = chunk_of(N, NB);
+    const int64_t o = (int64_t)j * N;
+    double v[4] = {0, 0, 0, 0};
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += kVT) {
+        double2 bi = b[o + i];
+        double2 ri = csub(bi, Ax[o + i]);
+        if (r) r[o + i] = ri;
+        v[0] += ri.x * ri.x + ri.y * ri.y;
+        v[1] += bi.x * bi.x + bi.y * bi.y;
+    }
+    block_reduce4(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
+}
+
+// ------------------------------------------------------------------ host driver
+namespace {
+
+struct Work {
+    DBuf buf;
+    double2 *r, *r0, *p, *v, *s, *t, *Ax;
+    double *part;
+    RhsState *S;
+    int *mask;
+    void init(int64_t N, int R, int NB) {
+        const int64_t vec = N * R;
+        const size_t vb = sizeof(double2) * vec;
+        const size_t pb = sizeof(double) * 4 * (size_t)R * NB;
+        size_t total = 7 * vb + 4 * pb + sizeof(RhsState) * R + sizeof(int) * R + 7 * 256;
+        buf.alloc(total);
+        char *c = buf.as<char>();
+        auto take = [&](size_t n) {
+            char *q = c;
+            c += (n + 255) / 256 * 256;
+            return q;
+        };
+        r = (double2 *)take(vb), r0 = (double2 *)take(vb), p = (double2 *)take(vb), v = (double2 *)take(vb);
+        s = (double2 *)take(vb), t = (double2 *)take(vb), Ax = (double2 *)take(vb);
+        part = (double *)take(4 * pb);
+        S = (RhsState *)take(sizeof(RhsState) * R);
+        mask = (int *)take(sizeof(int) * R);
+    }
+};
+
+int pick_nb(int N) { return std::max(1, std::min(kMaxNB, (N + 2047) / 2048)); }
+
+}  // namespace
+
+// BiCGSTAB on operator `prec` (ARH_FP64 / ARH_MIXED) for the rhs with want[j] != 0, from x = 0.
+// Returns per-rhs iterations; fills recursive residual history.
+static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 *b, double2 *x,
+                     const std::vector<int> &want, double tol, int maxit, const arh_solve_opts &o,
+                     arh_solve_stats *st, std::vector<int> &iters, std::vector<double> &err, Work &W, int NB,
+                     bool record_hist) {
+    arh_ctx *ctx = h->g->ctx;
+    hipStream_t s = ctx->stream;
+    const int N = h->g->N;
+    const dim3 gv(NB, R), bv(kVT);
+    const int pblk = (R + 63) / 64;
+    double *P1 = W.part, *P2 = W.part + 4 * (size_t)R * NB, *P3 = W.part + 8 * (size_t)R * NB;
+    std::vector<RhsState> hs(R);
+    for (int j = 0; j < R; j++) {
+        std::memset(&hs[j], 0, sizeof(RhsState));
+        hs[j].restart = want[j] ? 1 : 0;
+    }
+    ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, x, 1);
+    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, W.p, 1);
+    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, W.v, 1);
+    hipLaunchKernelGGL(k_restart, gv, bv, 0, s, W.S, N, NB, b, (const double2 *)nullptr, W.r, W.r0, P1);
+    hipLaunchKernelGGL(k_restart_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, 1);
+    ARH_HIP(hipGetLastError());
+    ARH_HIP(hipMemcpyAsync(hs.data(), W.S, sizeof(RhsState) * R, hipMemcpyDeviceToHost, s));
+    ARH_HIP(hipStreamSynchronize(s));
+    iters.assign(R, 0);
+    err.assign(R, 0.0);
+    std::vector<int> restarts(R, 0);
+    std::vector<double> tol2(R);
+    int nact = 0;
+    for (int j = 0; j < R; j++) {
+        hs[j].active = 0;
+        if (!want[j]) continue;
+        if (hs[j].rhs_sq == 0.0) {  // Eigen: x = 0, done
+            err[j] = 0.0;
+            continue;
+        }
+        tol2[j] = tol * tol * hs[j].rhs_sq;
+        if (record_hist && st && st->hist && st->hist_len && o.hist_cap > 0) {
+            int &hl = st->hist_len[j];
+            if (hl < o.hist_cap) st->hist[(size_t)j * o.hist_cap + hl++] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
+        }
+        if (hs[j].rn > tol2[j] && maxit > 0) hs[j].active = 1, nact++;
+        err[j] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
+    }
+    const double eps2 = std::numeric_limits<double>::epsilon() * std::numeric_limits<double>::epsilon();
+    std::vector<int> it(R, 0);
+    while (nact > 0) {
+        // restart check (top of Eigen's loop)
+        bool any_restart = false;
+        for (int j = 0; j < R; j++) {
+            hs[j].restart = 0;
+            if (hs[j].active && std::hypot(hs[j].rho_new.x, hs[j].rho_new.y) < eps2 * hs[j].r0sq) {
+                hs[j].restart = 1;
+                any_restart = true;
+            }
+        }
+        ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
+        if (any_restart) {
+            hmat_apply(h, transpose, prec, R, x, W.Ax, s);
+            if (st) st->matvecs++;
+            hipLaunchKernelGGL(k_restart, gv, bv, 0, s, W.S, N, NB, b, W.Ax, W.r, W.r0, P1);
+            hipLaunchKernelGGL(k_restart_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, 0);
+            for (int j = 0; j < R; j++)
+                if (hs[j].restart) {
+                    if (restarts[j]++ == 0) it[j] = 0;
+                    if (st) st->restarts++;
+                }
+        }
+        hipLaunchKernelGGL(k_p_update, gv, bv, 0, s, W.S, N, NB, W.r, W.p, W.v);
+        hmat_apply(h, transpose, prec, R, W.p, W.v, s);
+        hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.r0, W.v, (const double2 *)nullptr, P1);
+        hipLaunchKernelGGL(k_s_update, gv, bv, 0, s, W.S, N, NB, P1, W.r, W.v, W.s);
+        hmat_apply(h, transpose, prec, R, W.s, W.t, s);
+        hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.t, W.s, W.t, P2);
+        hipLaunchKernelGGL(k_xr_update, gv, bv, 0, s, W.S, N, NB, P1, P2, W.p, W.s, W.t, W.r0, x, W.r, P3);
+        hipLaunchKernelGGL(k_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, P2, P3);
+        ARH_HIP(hipGetLastError());
+        if (st) st->matvecs += 2;
+        ARH_HIP(hipMemcpyAsync(hs.data(), W.S, sizeof(RhsState) * R, hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipStreamSynchronize(s));
+        if (ctx->prof.on) ctx->prof.flush(false);
+        nact = 0;
+        for (int j = 0; j < R; j++) {
+            if (!hs[j].active) continue;
+            it[j]++;
+            iters[j]++;
+            err[j] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
+            if (record_hist && st && st->hist && st->hist_len && o.hist_cap > 0) {
+                int &hl = st->hist_len[j];
+                if (hl < o.hist_cap) st->hist[(size_t)j * o.hist_cap + hl++] = err[j];
+            }
+            if (hs[j].rn > tol2[j] && it[j] < maxit)
+                nact++;
+            else
+                hs[j].active = 0;
+        }
+    }
+}
+
+// true residual ||b - A x|| / ||b|| per rhs (FP64 operator); r receives b - A x if non-null
+static void true_residual(arh_hmat *h, int transpose, int R, const double2 *b, const double2 *x, double2 *r, Work &W,
+                          int NB, std::vector<double> &res, arh_solve_stats *st) {
+    hipStream_t s = h->g->ctx->stream;
+    const int N = h->g->N;
+    hmat_apply(h, transpose, ARH_FP64, R, x, W.Ax, s);
+    if (st) st->matvecs++;
+    hipLaunchKernelGGL(k_resid, dim3(NB, R), dim3(kVT), 0, s, N, NB, b, W.Ax, r, W.part);
+    ARH_HIP(hipGetLastError());
+    std::vector<double> part(4 * (size_t)R * NB);
+    ARH_HIP(hipMemcpyAsync(part.data(), W.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    ARH_HIP(hipStreamSynchronize(s));
+    res.assign(R, 0.0);
+    for (int j = 0; j < R; j++) {
+        double rr = 0, bb = 0;
+        for (int q = 0; q < NB; q++) rr += part[((size_t)j * NB + q) * 4], bb += part[((size_t)j * NB + q) * 4 + 1];
+        res[j] = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    }
+}
+
+void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, const arh_solve_opts &o,
+           arh_solve_stats *st) {
+    arh_geom *g = h->g;
+    hipStream_t s = g->ctx->stream;
+    const int N = g->N;
+    const int NB = pick_nb(N);
+    const int maxit = o.maxit > 0 ? o.maxit : 2 * N;
+    const double tol = o.tol;
+    ARH_CHECK(R >= 1, ARH_EINVAL, "nrhs must be >= 1");
+    ARH_CHECK(tol >= 0.0, ARH_EINVAL, "tol must be >= 0");
+    Work W;
+    W.init(N, R, NB);
+    if (st) {
+        st->iters = st->matvecs = st->refinements = st->restarts = 0;
+        st->max_err = st->max_true_res = 0.0;
+        if (st->hist_len)
+            for (int j = 0; j < R; j++) st->hist_len[j] = 0;
+    }
+    std::vector<int> want(R, 1), iters, iters_tot(R, 0);
+    std::vector<double> err, tres;
+    if (h->storage != ARH_MIXED) {
+        bicgstab(h, transpose, ARH_FP64, R, b, x, want, tol, maxit, o, st, iters, err, W, NB, true);
+        for (int j = 0; j < R; j++) iters_tot[j] = iters[j];
+        // optional: continue from x until the FP64 true residual meets tol
+        int sweeps = 0;
+        std::vector<double> last_err = err;
+        if (o.true_residual || st) true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
+        while (o.true_residual && sweeps < 8) {
+            std::vector<int> redo(R, 0);
+            int nre = 0;
+            for (int j = 0; j < R; j++)
+                if (tres[j] > tol) redo[j] = 1, nre++;
+            if (!nre) break;
+            // correction solve A d = b - A x for the unconverged rhs, then x += d
+            DBuf rb, dx;
+            rb.alloc(sizeof(double2) * (size_t)N * R);
+            dx.alloc(sizeof(double2) * (size_t)N * R);
+            true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, nullptr);
+            std::vector<double> rel(R);
+            double ctol = tol;
+            for (int j = 0; j < R; j++)
+                if (redo[j]) ctol = std::min(ctol, tol / std::max(tres[j], 1e-300) * 0.5);
+            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), redo, std::min(0.5, ctol), maxit,
+                     o, st, iters, err, W, NB, false);
+            ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_axpy_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s,
+                               W.mask, N, x, dx.as<double2>());
+            ARH_HIP(hipGetLastError());
+            for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
+            true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
+            sweeps++;
+            if (st) st->refinements++;
+        }
+        err = last_err;
+    } else {
+        // mixed: FP32-operator BiCGSTAB inside FP64 true-residual refinement
+        DBuf rb, dx;
+        rb.alloc(sizeof(double2) * (size_t)N * R);
+        dx.alloc(sizeof(double2) * (size_t)N * R);
+        ARH_HIP(hipMemcpyAsync(rb.p, b, sizeof(double2) * (size_t)N * R, hipMemcpyDeviceToDevice, s));
+        ARH_HIP(hipMemsetAsync(x, 0, sizeof(double2) * (size_t)N * R, s));
+        std::vector<double> bnorm_rel(R, 1.0);  // ||r|| / ||b|| of the current residual
+        std::vector<int> redo(R, 1);
+        for (int sweep = 0; sweep < 10; sweep++) {
+            // inner tolerance relative to the current residual; aim at half the target
+            double itol = 1.0;
+            for (int j = 0; j < R; j++)
+                if (redo[j]) itol = std::min(itol, 0.5 * tol / std::max(bnorm_rel[j], 1e-300));
+            itol = std::max(itol, 1e-7);
+            bicgstab(h, transpose, ARH_MIXED, R, rb.as<double2>(), dx.as<double2>(), redo, std::min(itol, 0.5), maxit,
+                     o, st, iters, err, W, NB, sweep == 0);
+            for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
+            ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_axpy_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s,
+                               W.mask, N, x, dx.as<double2>());
+            ARH_HIP(hipGetLastError());
+            true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
+            if (st) st->refinements++;
+            int nre = 0;
+            for (int j = 0; j < R; j++) {
+                bnorm_rel[j] = tres[j];
+                redo[j] = tres[j] > tol ? 1 : 0;
+                nre += redo[j];
+            }
+            if (!nre) break;
+        }
+        err = tres;
+    }
+    if (tres.empty()) true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
+    if (st) {
+        for (int j = 0; j < R; j++) {
+            st->iters = std::max(st->iters, iters_tot[j]);
+            st->max_err = std::max(st->max_err, err[j]);
+            st->max_true_res = std::max(st->max_true_res, tres[j]);
+            if (st->iters_per_rhs) st->iters_per_rhs[j] = iters_tot[j];
+            if (st->true_res) st->true_res[j] = tres[j];
+        }
+    }
+    if (o.fail_on_noconv) {
+        for (int j = 0; j < R; j++)
+            ARH_CHECK(tres[j] <= tol * (o.true_residual || h->storage == ARH_MIXED ? 1.0 : 10.0), ARH_ECONV,
+                      "BiCGSTAB did not reach the tolerance (rhs " + std::to_string(j) +
+                          ", true residual " + std::to_string(tres[j]) + ")");
+    }
+}
+
+}  // namespace arh
