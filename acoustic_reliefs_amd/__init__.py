@@ -10,8 +10,9 @@ _HERE = _os.path.dirname(_os.path.abspath(__file__))
 
 
 def _load():
+    import importlib
     try:
-        from . import acoustics3d_impl as impl  # noqa: F401
+        impl = importlib.import_module(__name__ + ".acoustics3d_impl")
     except ImportError as e:  # pragma: no cover - exercised when not built
         raise ImportError("acoustic_reliefs_amd: the native extension is not built (run __graft_entry__.build()): %s"
                           % e) from e

@@ -145,6 +145,12 @@ struct arh_ctx_s {
     void *pinned(size_t bytes);
 };
 
+// FP64 product work unit: up to 64 consecutive columns of one row leaf's column list, taken from
+// its dense panel (kind 0) or from the leaf's segment of low-rank block blk's U (kind 1).
+struct MvUnit {
+    int leaf, kind, blk, c0, nc;
+};
+
 // Dense-leaf assembly task: row leaf, first panel column, column count (cols * m <= 256).
 struct DenseTask {
     int leaf, c0, nc, pad;
@@ -174,7 +180,7 @@ struct arh_geom_s {
     std::vector<int> lowrank;  // na x 4
     int nd = 0, na = 0;
     // leaves
-    int nleaf = 0;
+    int nleaf = 0, max_leaf = 0;
     std::vector<int> leaf_start;  // nleaf + 1
     std::vector<int> leaf_of;     // N
     arh::DBuf d_leaf_start;
@@ -216,6 +222,14 @@ struct arh_hmat_s {
     std::vector<int> active_lr;  // lr blocks with rank > 0
     arh::DBuf d_active_lr;
     arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
+    // MFMA multi-rhs product: packed FP32 rhs, FP32 intermediates, phase-1 work units (block, row tile)
+    arh::DBuf xf32, t32, d_p1units;
+    std::vector<int2> p1units;
+    // FP64 product: phase-1 units (block, 4-row group), phase-2 units per leaf, per-unit partials
+    std::vector<MvUnit> mv_units;
+    std::vector<int> mv_uptr;
+    std::vector<int2> p1units64;
+    arh::DBuf d_mv_units, d_mv_uptr, d_p1units64, mv_part;
     arh_hmat_info info{};
 };
 
@@ -228,6 +242,11 @@ void assemble_aca(arh_hmat *h);
 void make_fp32_copy(arh_hmat *h);
 // matvec.hip
 void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+// matvec_mfma.hip
+void hmat_apply_mfma(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+// matvec64.hip
+void hmat_apply64(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+int mfma_max_leaf();
 // rhs / receivers / diffusion (receivers.hip)
 void launch_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs_dev, double2 *b);
 void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int transpose, int nrhs, const double2 *x,

@@ -60,6 +60,8 @@ void geom_build_index(arh_geom *g) {
     g->nleaf = (int)g->leaf_start.size() - 1;
     const int nl = g->nleaf;
     g->leaf_of.assign(N, -1);
+    g->max_leaf = 0;
+    for (int l = 0; l < nl; l++) g->max_leaf = std::max(g->max_leaf, g->leaf_start[l + 1] - g->leaf_start[l]);
     for (int l = 0; l < nl; l++)
         for (int i = g->leaf_start[l]; i < g->leaf_start[l + 1]; i++) g->leaf_of[i] = l;
     auto leaf_exact = [&](int a, int b) {  // [a, b) is exactly one leaf

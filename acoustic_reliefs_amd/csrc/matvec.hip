@@ -286,8 +286,16 @@ void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const doubl
     ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
     if (precision == ARH_MIXED) {
         ARH_CHECK(h->storage == ARH_MIXED, ARH_ESTATE, "FP32 product requested but the H-matrix has no FP32 copy");
+        if (!transpose && nrhs >= 4 && h->g->max_leaf <= mfma_max_leaf()) {
+            hmat_apply_mfma(h, nrhs, x, y, s);
+            return;
+        }
         apply_t<float2>(h, transpose, nrhs, h->dense32.as<float2>(), h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
     } else {
+        if (!transpose) {
+            hmat_apply64(h, nrhs, x, y, s);
+            return;
+        }
         apply_t<double2>(h, transpose, nrhs, h->dense.as<double2>(), h->U.as<double2>(), h->V.as<double2>(), x, y, s);
     }
 }

@@ -382,3 +382,43 @@ def sphere_scattered(k, a, r, cos_theta, nterms=None):
         jr, yr = _sph(n, k * r)
         val -= (1j ** n) * (2 * n + 1) * (dja / (dja + 1j * dya)) * eval_legendre(n, ct) * (jr + 1j * yr)
     return np.conj(val)
+
+
+# ----------------------------------------------------------------- DiffBEM-equivalent band pipeline
+def band_value_grad(oc, ks, L, diff=None, src=(0.0, 100.0, 0.0), aca_tol=1e-3, solver_tol=1e-3, want_grad=False,
+                    use_actual=False, src_kind=0, directions=None):
+    """bem3d::gradient for one band (adjoint.cpp:24-242) with the exact listener matrix.
+
+    ks: wavenumbers of the band's frequencies. src_kind 0: point source src; 1: plane wave
+    src (a direction). Returns d, or (d, dd/dh) when want_grad.
+    """
+    states = []
+    yt = np.zeros(len(L))
+    for k in ks:
+        b = oc.rhs_point(k, src) if src_kind == 0 else oc.rhs_plane(k, src)
+        if use_actual:
+            A = oc.dense_matrix(k)
+            x = dense_solve(A, -b)
+        else:
+            A = None
+            oc.assemble(k, aca_tol, 100)
+            x, _, _, _ = oc.bicgstab(-b, solver_tol)
+        Q = oc.listener_matrix(k, L)
+        y = Q @ x
+        yt += np.abs(y)
+        states.append((k, A, Q, x, y))
+    d, u = diffusion(yt)
+    if not want_grad:
+        return d
+    g = np.zeros(len(diff))
+    for k, A, Q, x, y in states:
+        vf = u * np.conj(y) / np.abs(y)
+        adj = Q.T @ vf
+        if use_actual:
+            lam = dense_solve(A.T.copy(), adj)
+        else:
+            oc.assemble(k, aca_tol, 100)
+            lam, _, _, _ = oc.bicgstab(adj, solver_tol, transpose=True)
+        gA, gB, gC = oc.shape_grad(k, L, diff, x, lam, vf, src_kind, src)
+        g += gA - gB + gC
+    return d, g

@@ -1,0 +1,68 @@
+"""torch.autograd.Function over the relief workload (FP64 and mixed precision), checked against
+central finite differences (h = 1e-5 m) and the oracle's adjoint gradient for the same plane waves."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(small_relief, storage, recompute=False):
+    from acoustic_reliefs_amd import autograd, capi, meshes
+    P, E, top = small_relief
+    return autograd.ReliefBEM(P, E, top, bands=[630.0, 1250.0], directions=meshes.incidence_directions(5),
+                              receivers=meshes.semicircle_receivers(5.0, 37), leaf=16, aca_tol=1e-8, tol=1e-10,
+                              storage=storage, recompute=recompute)
+
+
+@pytest.mark.parametrize("storage_name", ["fp64", "mixed"])
+def test_autograd_vs_fd(small_relief, capi, storage_name):
+    import torch
+    from acoustic_reliefs_amd import autograd
+    storage = capi.ARH_FP64 if storage_name == "fp64" else capi.ARH_MIXED
+    eng = _engine(small_relief, storage)
+    HH = len(eng.diff)
+    h = torch.tensor(0.01 * np.sin(np.arange(HH) * 0.7), dtype=torch.float64, requires_grad=True)
+    d = autograd.diffusion_coefficients(h, eng)
+    loss = (1.0 - d).mean()  # mean over bands of (1 - d_b)
+    loss.backward()
+    g = h.grad.numpy()
+    rng = np.random.default_rng(1)
+    pick = rng.choice(HH, 4, replace=False)
+    step = 1e-5
+    h0 = h.detach().numpy()
+    for t in pick:
+        hp, hm = h0.copy(), h0.copy()
+        hp[t] += step
+        hm[t] -= step
+        lp = np.mean(1.0 - eng.forward(hp)[0])
+        lm = np.mean(1.0 - eng.forward(hm)[0])
+        fd = (lp - lm) / (2 * step)
+        print("%s vertex %d: fd %.6e adjoint %.6e" % (storage_name, t, fd, g[t]))
+        tol = 2e-4 if storage_name == "fp64" else 5e-3
+        assert abs(fd - g[t]) <= tol * max(abs(fd), 1e-3)
+
+
+def test_autograd_vs_oracle_and_recompute(small_relief, oracle):
+    eng = _engine(small_relief, 0)
+    P, E, top = small_relief
+    HH = len(eng.diff)
+    h = 0.01 * np.cos(np.arange(HH) * 0.3)
+    d_b, d_all = eng.forward(h)
+    g = eng.backward(np.array([1.0, 0.0]))
+    Pd = P.copy()
+    Pd[top, 1] += h
+    oc = oracle.Ctx(Pd, eng.g.E, eng.g.dense, eng.g.lowrank)
+    k = eng.k(630.0)
+    go = np.zeros(HH)
+    do = []
+    for dvec in eng.dirs:
+        dv, gv = oracle.band_value_grad(oc, [k], eng.L, top, src=dvec, src_kind=1, aca_tol=1e-8, solver_tol=1e-10,
+                                        want_grad=True)
+        do.append(dv)
+        go += gv / len(eng.dirs)
+    assert np.abs(np.array(do) - d_all[0]).max() < 1e-7
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 1e-5
+    eng2 = _engine(small_relief, 0, recompute=True)
+    d2, _ = eng2.forward(h)
+    g2 = eng2.backward(np.array([1.0, 0.0]))
+    assert np.array_equal(d2, d_b) and np.allclose(g2, g, rtol=1e-9, atol=0)

@@ -1,0 +1,271 @@
+"""Pins the CPU oracle (oracle/) against what the mathematics fixes.
+
+The reference ships no golden vectors and cannot be built here (its Eigen submodule is empty),
+so each oracle function is checked against closed forms, invariants, brute force and the
+analytic rigid-sphere series, chosen so that a dropped term, a wrong sign or a transposed
+operand fails one of them.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+K500 = 2 * np.pi * 500.0 / 343.0
+
+
+def _tri():
+    return np.array([[0.1, 0.0, 0.2], [0.5, 0.05, 0.1], [0.2, -0.1, 0.6]])
+
+
+def _sub_entry(O, k, p, tri, levels):
+    """Brute force: the same 6-point rule on 4**levels sub-triangles (converges to the integral)."""
+    tris = [tri]
+    for _ in range(levels):
+        nt = []
+        for a, b, c in tris:
+            ab, bc, ca = (a + b) / 2, (b + c) / 2, (c + a) / 2
+            nt += [np.array([a, ab, ca]), np.array([ab, b, bc]), np.array([ca, bc, c]), np.array([ab, bc, ca])]
+        tris = nt
+    return sum(O.entry(k, p, t) for t in tris)
+
+
+def test_entry_brute_force(oracle):
+    """Against the rule on 256 sub-triangles: tiny error for a smooth integrand, and the error of
+    a degree-4 rule shrinks faster than h^4 when the triangle is halved (a wrong weight, point or
+    normal scaling gives an O(1) error that does not shrink)."""
+    tri = _tri()
+    p = np.array([3.0, 2.0, -1.0])
+    e0, e4 = oracle.entry(1.0, p, tri), _sub_entry(oracle, 1.0, p, tri, 4)
+    assert abs(e0 - e4) / abs(e4) < 1e-6
+    c = tri.mean(0)
+    errs = []
+    for s in (0.2, 0.1):
+        t = c + s * (tri - c)
+        errs.append(abs(oracle.entry(9.0, p, t) - _sub_entry(oracle, 9.0, p, t, 4)) / abs(_sub_entry(oracle, 9.0, p, t, 4)))
+    assert errs[1] < errs[0] / 16
+
+
+def test_entry_far_field_solid_angle(oracle):
+    """k = 0, p far along +n: M -> +A / (4 pi R^2), the triangle's solid angle over 4 pi seen from
+    the side its normal points to (fixes the sign, the 1/2 rule factor and |n| = 2A)."""
+    tri = _tri()
+    n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    c = tri.mean(0)
+    area = 0.5 * np.linalg.norm(n)
+    for side in (1.0, -1.0):
+        p = c + side * 1e4 * n / np.linalg.norm(n)
+        m = oracle.entry(0.0, p, tri)
+        assert abs(m.imag) < 1e-20
+        assert abs(m.real - side * area / (4 * np.pi * 1e8)) < 1e-6 * area / (4 * np.pi * 1e8)
+
+
+def test_self_entry_zero_on_flat_panel(oracle):
+    tri = _tri()
+    c = tri.mean(0)
+    assert abs(oracle.entry(K500, c, tri)) < 1e-12
+
+
+def test_gauss_law_sphere(oracle, sphere_mesh):
+    """k = 0: sum_j M_ij ~ -1/2 on a closed surface with outward normals; Q rows ~ 0 outside."""
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    A = oc.dense_matrix(0.0) + 0.5 * np.eye(len(Er))
+    s = A.sum(1)
+    assert np.abs(s.imag).max() == 0
+    assert np.abs(s.real + 0.5).max() < 2e-3
+    L = np.array([[2.0, 0.3, 0.1], [0.0, -3.0, 1.0]])
+    Q = oc.listener_matrix(0.0, L)
+    assert np.abs(Q.sum(1)).max() < 1e-4
+    # interior point: solid angle -1
+    Qi = oc.listener_matrix(0.0, np.array([[0.05, -0.02, 0.1]]))
+    assert abs(Qi.sum() + 1.0) < 1e-3
+
+
+def test_partition_invariants(oracle, sphere_mesh, small_relief):
+    for P, E in (sphere_mesh, small_relief[:2]):
+        C, _ = oracle.intermediates(P, E)
+        for leaf in (8, 32):
+            perm, d, a, leaves = oracle.partition(C, leaf, 1.5)
+            N = len(E)
+            assert np.array_equal(np.sort(perm), np.arange(N))
+            cover = np.zeros((N, N), np.int32)
+            for rs, re, cs, ce in np.vstack([d, a]):
+                cover[rs:re, cs:ce] += 1
+            assert (cover == 1).all()
+            assert oracle.tree_check(C, leaf)
+            assert (leaves[:, 1] - leaves[:, 0] <= leaf).all()
+            for rs, re, cs, ce in d:
+                assert re - rs <= leaf and ce - cs <= leaf
+
+
+def test_partition_reference_counts(oracle, sphere_mesh):
+    P, E = sphere_mesh
+    C, _ = oracle.intermediates(P, E)
+    _, d, a, _ = oracle.partition(C, 32, 1.5)
+    # the same counts a throwaway probe of the reference rules gave for this mesh
+    assert len(d) == 735 and len(a) == 1266
+
+
+def test_aca_error_bound_and_svd(oracle, sphere_mesh):
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    for tol in (1e-3, 1e-6):
+        oc.assemble(K500, tol, 100)
+        A = oc.dense_matrix(K500) + 0.5 * np.eye(len(Er))
+        ranks = oc.ranks()
+        for blk in range(0, len(appb), 97):
+            rs, re, cs, ce = appb[blk]
+            B = A[rs:re, cs:ce]
+            U, V = oc.factors(blk)
+            assert U.shape[1] == ranks[blk]
+            err = np.linalg.norm(B - U @ V) / np.linalg.norm(B)
+            assert err < 30 * tol, (blk, err)
+            sv = np.linalg.svd(B, compute_uv=False)
+            # ACA can not beat the optimal rank-r error
+            if ranks[blk] < min(B.shape):
+                assert err >= 0.99 * sv[ranks[blk]] / np.linalg.norm(sv) - 1e-15
+
+
+def test_hmatvec_vs_dense(oracle, sphere_mesh):
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    oc.assemble(K500, 1e-8, 100)
+    A = oc.dense_matrix(K500)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(len(Er)) + 1j * rng.standard_normal(len(Er))
+    assert rel(oc.matvec(x), A @ x) < 1e-7
+    assert rel(oc.matvec(x, transpose=True), A.T @ x) < 1e-7  # transpose, not conjugate transpose
+
+
+def test_bicgstab_matches_dense_and_tolerance(oracle, sphere_mesh):
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    oc.assemble(K500, 1e-10, 100)
+    A = oc.dense_matrix(K500)
+    b = oc.rhs_plane(K500, [0.0, 0.0, -1.0])
+    xd = np.linalg.solve(A, -b)
+    assert rel(oracle.dense_solve(A, -b), xd) < 1e-12
+    for tol in (1e-4, 1e-10):
+        x, it, err, hist = oc.bicgstab(-b, tol)
+        assert err <= tol and len(hist) == it + 1 and hist[0] == 1.0
+        # the solver's own operator (the H-matrix) meets the tolerance; the dense matrix differs by the ACA error
+        assert np.linalg.norm(oc.matvec(x) + b) / np.linalg.norm(b) < 10 * tol
+        assert rel(x, xd) < 100 * tol + 1e-7
+    # transposed solve (adjoint.cpp:201-209)
+    x, it, err, _ = oc.bicgstab(b, 1e-10, transpose=True)
+    assert rel(x, np.linalg.solve(A.T, b)) < 1e-8
+    # zero right-hand side: x = 0 without iterating (Eigen's early exit)
+    x, it, err, _ = oc.bicgstab(np.zeros_like(b), 1e-6)
+    assert it == 0 and not np.any(x)
+
+
+def test_diffusion_closed_forms(oracle):
+    d, u = oracle.diffusion(np.full(181, 0.7))
+    assert abs(d - 1.0) < 1e-14
+    assert np.abs(u).max() < 1e-12
+    yt = np.zeros(50)
+    yt[7] = 2.0
+    d, _ = oracle.diffusion(yt)
+    assert abs(d) < 1e-15
+    rng = np.random.default_rng(1)
+    yt = rng.random(30) + 0.1
+    d, u = oracle.diffusion(yt)
+    h = 1e-6
+    for l in (0, 13, 29):
+        e = np.zeros(30)
+        e[l] = h
+        fd = (oracle.diffusion(yt + e)[0] - oracle.diffusion(yt - e)[0]) / (2 * h)
+        assert abs(fd - u[l]) < 1e-7 * max(1.0, abs(u[l]))
+
+
+def test_entry_derivatives_fd(oracle):
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    for trial in range(4):
+        ptri = rng.random((3, 3)) + np.array([2.0, 0.0, 0.0])
+        tri = rng.random((3, 3))
+        k = 3.0 + 10 * trial
+        # collocation element moves (any vertex moves the centroid by 1/3)
+        d0 = oracle.dentry(k, ptri, tri, 0)
+        for v in range(3):
+            pp, pm = ptri.copy(), ptri.copy()
+            pp[v, 1] += h
+            pm[v, 1] -= h
+            fd = (oracle.entry(k, pp.mean(0), tri) - oracle.entry(k, pm.mean(0), tri)) / (2 * h)
+            assert abs(fd - d0) < 1e-6 * max(abs(d0), 1e-3), (trial, v)
+        # integration element corner moves (the normal's length and direction change too)
+        for c in range(3):
+            tp, tm = tri.copy(), tri.copy()
+            tp[c, 1] += h
+            tm[c, 1] -= h
+            fd = (oracle.entry(k, ptri.mean(0), tp) - oracle.entry(k, ptri.mean(0), tm)) / (2 * h)
+            dc = oracle.dentry(k, ptri, tri, c + 1)
+            assert abs(fd - dc) < 1e-6 * max(abs(dc), 1e-3), (trial, c)
+
+
+@pytest.mark.parametrize("src_kind", [0, 1])
+def test_shape_gradient_fd(oracle, src_kind):
+    """Adjoint gradient (A - B + C) vs central differences of the whole forward (dense path)."""
+    from acoustic_reliefs_amd import meshes
+    h = meshes.smooth_heights(6, seed=2, sigma=1.0, hmax=0.05)
+    P, E, top = meshes.relief_box(width=0.3, depth=0.3, base=0.05, grid=6, heights=h, wall_rows=1, bottom_grid=2)
+    Er, dirb, appb, _ = oracle.setup(P, E, 8, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    L = oracle.listener_points(5.0, 30)
+    src = (0.0, 10.0, 2.0) if src_kind == 0 else (0.3, -0.9, 0.1)
+    src = np.array(src) / (1 if src_kind == 0 else np.linalg.norm(src))
+    ks = [2 * np.pi * f / 343.0 for f in (700.0, 1100.0)]
+    diff = top[[3, 10, 21, 28]]
+    d, g = oracle.band_value_grad(oc, ks, L, diff, src=src, want_grad=True, use_actual=True, src_kind=src_kind)
+    step = 1e-6
+    for t, v in enumerate(diff):
+        Pp, Pm = P.copy(), P.copy()
+        Pp[v, 1] += step
+        Pm[v, 1] -= step
+        oc.set_vertices(Pp)
+        dp = oracle.band_value_grad(oc, ks, L, src=src, use_actual=True, src_kind=src_kind)
+        oc.set_vertices(Pm)
+        dm = oracle.band_value_grad(oc, ks, L, src=src, use_actual=True, src_kind=src_kind)
+        oc.set_vertices(P)
+        fd = (dp - dm) / (2 * step)
+        assert abs(fd - g[t]) < 1e-5 * max(abs(fd), 1e-4), (t, fd, g[t])
+
+
+def test_analytic_series_consistency(oracle):
+    """Incident + scattered series at r = a equals the surface series (Wronskian identity)."""
+    ka = 4.58
+    ct = np.linspace(-1, 1, 9)
+    k, a = ka / 0.5, 0.5
+    surf = oracle.sphere_surface_pressure(ka, ct)
+    scat = oracle.sphere_scattered(k, a, a, ct)
+    # incident e^{-ik d.x} at the point with cos(theta) = ct from the propagation direction
+    inc = np.exp(-1j * k * a * ct)
+    assert np.abs(inc + scat - surf).max() < 1e-10
+
+
+def test_sphere_solution_vs_analytic(oracle, sphere_mesh):
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    oc.assemble(K500, 1e-4, 100)
+    b = oc.rhs_plane(K500, [0.0, 0.0, -1.0])
+    x, it, err, _ = oc.bicgstab(-b, 1e-6)
+    C, _ = oracle.intermediates(P, Er)
+    ct = -C[:, 2] / np.linalg.norm(C, axis=1)
+    pa = oracle.sphere_surface_pressure(K500 * 0.5, ct)
+    e3 = rel(x, pa)
+    assert e3 < 0.015
+    # refinement converges: 4x the triangles -> smaller error
+    from acoustic_reliefs_amd import meshes
+    P4, E4 = meshes.icosphere(4, 0.5)
+    Er4, d4, a4, _ = oracle.setup(P4, E4, 32, 1.5)
+    oc4 = oracle.Ctx(P4, Er4, d4, a4)
+    oc4.assemble(K500, 1e-4, 100)
+    x4, _, _, _ = oc4.bicgstab(-oc4.rhs_plane(K500, [0.0, 0.0, -1.0]), 1e-6)
+    C4, _ = oracle.intermediates(P4, Er4)
+    e4 = rel(x4, oracle.sphere_surface_pressure(K500 * 0.5, -C4[:, 2] / np.linalg.norm(C4, axis=1)))
+    assert e4 < 0.6 * e3
