@@ -30,9 +30,12 @@ class ReliefBEM:
     """Device state of one relief: geometry, receivers, incidences, bands and solver settings."""
 
     def __init__(self, P, E, diff, bands, directions, receivers, c0=343.0, leaf=32, factor=1.5, aca_tol=1e-4,
-                 tol=1e-6, storage=capi.ARH_FP64, recompute=False, device=0, ctx=None):
+                 tol=1e-6, storage=capi.ARH_FP64, recompute=False, device=0, ctx=None, all_dense=False):
+        """all_dense=True: every leaf pair dense (the exact operator). ACA pivots and ranks change
+        discontinuously with the vertex positions, so finite differences are only meaningful
+        against a forward without ACA."""
         self.ctx = ctx or capi.Context(device)
-        self.g = capi.Geometry.from_mesh(self.ctx, P, E, leaf, factor)
+        self.g = capi.Geometry.from_mesh(self.ctx, P, E, leaf, factor, all_dense=all_dense)
         self.P0 = np.array(P, dtype=np.float64)
         self.diff = np.ascontiguousarray(diff, dtype=np.int32)
         self.bands = [float(b) for b in bands]

@@ -62,6 +62,18 @@ class Profile(ctypes.Structure):
         return {f: getattr(self, f).as_dict() for f, _ in self._fields_}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("workers", c_int), ("hbm_budget", ctypes.c_int64), ("device", c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", c_int), ("nrhs", c_int), ("t_submit", c_double), ("t_start", c_double),
+                ("t_done", c_double), ("footprint", ctypes.c_int64), ("iters", c_int), ("max_true_res", c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every entry point declared in include/arh.h
 SIGNATURES = [
     ("arh_ctx_create", c_int, [c_int, ctypes.POINTER(c_void_p)]),
@@ -97,6 +109,13 @@ SIGNATURES = [
     ("arh_band_solve", c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, P_D, P_D, c_int,
                                ctypes.POINTER(SolveOpts), P_D, c_void_p, c_void_p, ctypes.POINTER(SolveStats),
                                ctypes.POINTER(HMatInfo)]),
+    ("arh_server_create", c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(c_void_p)]),
+    ("arh_server_add_mesh", c_int, [c_void_p, P_D, c_int, P_I, c_int, c_int, c_double, P_I]),
+    ("arh_server_submit", c_int, [c_void_p, c_int, c_double, c_double, c_int, c_int, c_int, P_D, P_D, c_int, c_double,
+                                  ctypes.POINTER(ctypes.c_int64)]),
+    ("arh_server_wait", c_int, [c_void_p, ctypes.c_int64, P_D, ctypes.POINTER(RequestInfo)]),
+    ("arh_server_poll", c_int, [c_void_p, ctypes.c_int64, ctypes.POINTER(RequestInfo)]),
+    ("arh_server_destroy", None, [c_void_p]),
 ]
 
 _lib = None
@@ -263,9 +282,16 @@ class Geometry:
         self.h = h
 
     @classmethod
-    def from_mesh(cls, ctx, P, E, leaf=32, factor=1.5):
-        """set_mesh semantics: partition on the input order, reorder the elements."""
+    def from_mesh(cls, ctx, P, E, leaf=32, factor=1.5, all_dense=False):
+        """set_mesh semantics: partition on the input order, reorder the elements.
+
+        all_dense=True keeps the cluster tree's leaves but makes every leaf pair a dense block: the
+        exact operator (the use_actual path), whose solution is smooth in the vertex positions."""
         perm, dense, lowrank = partition(centroids(P, E), leaf, factor)
+        if all_dense:
+            leaves = dense[dense[:, 0] == dense[:, 2]][:, :2]
+            dense = np.array([[r[0], r[1], c[0], c[1]] for r in leaves for c in leaves], dtype=np.int32)
+            lowrank = np.zeros((0, 4), np.int32)
         g = cls(ctx, P, np.asarray(E, dtype=np.int32)[perm], dense, lowrank)
         g.perm = perm
         return g
@@ -409,3 +435,50 @@ class HMatrix:
         xd = self.geom.ctx.empty(b.shape)
         st = self.solve(bd, xd, nrhs, transpose, **kw)
         return xd.numpy(), st
+
+
+class Server:
+    """Continuously batched band solves (arh_server_*): submit returns at once, wait blocks."""
+
+    def __init__(self, workers=4, hbm_budget=0, device=0):
+        h = c_void_p()
+        opts = ServerOpts(workers=int(workers), hbm_budget=int(hbm_budget), device=int(device))
+        _check(lib().arh_server_create(ctypes.byref(opts), ctypes.byref(h)))
+        self.h = h
+        self._nrhs = {}
+
+    def add_mesh(self, P, E, leaf=32, factor=1.5):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.int32)
+        mid = c_int()
+        _check(lib().arh_server_add_mesh(self.h, _dp(P), len(P), _ip(E), len(E), int(leaf), float(factor),
+                                         ctypes.byref(mid)))
+        return mid.value
+
+    def submit(self, mesh, k, vecs, L, kind=ARH_RHS_PLANE, storage=ARH_MIXED, aca_tol=1e-4, tol=1e-6):
+        vecs = np.ascontiguousarray(np.asarray(vecs, dtype=np.float64).reshape(-1, 3))
+        L = np.ascontiguousarray(np.asarray(L, dtype=np.float64).reshape(-1, 3))
+        rid = ctypes.c_int64()
+        _check(lib().arh_server_submit(self.h, int(mesh), float(k), float(aca_tol), int(storage), int(kind), len(vecs),
+                                       _dp(vecs), _dp(L), len(L), float(tol), ctypes.byref(rid)))
+        self._nrhs[rid.value] = len(vecs)
+        return rid.value
+
+    def wait(self, rid):
+        d = np.zeros(self._nrhs[rid])
+        info = RequestInfo()
+        _check(lib().arh_server_wait(self.h, ctypes.c_int64(rid), _dp(d), ctypes.byref(info)))
+        return d, info.as_dict()
+
+    def poll(self, rid):
+        info = RequestInfo()
+        _check(lib().arh_server_poll(self.h, ctypes.c_int64(rid), ctypes.byref(info)))
+        return info.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None) and not _EXITING:
+            lib().arh_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()

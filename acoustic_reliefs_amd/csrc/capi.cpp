@@ -18,6 +18,10 @@ using namespace arh;
 
 static thread_local std::string tl_err;
 
+namespace arh {
+void set_thread_error(const std::string &m) { tl_err = m; }
+}  // namespace arh
+
 template <class F>
 static int guard(arh_ctx *ctx, F &&f) {
     try {

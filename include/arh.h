@@ -209,6 +209,40 @@ int arh_band_solve(arh_geom *g, double k, double aca_tol, int max_rank, int stor
                    const double *vecs, const double *L, int LL, const arh_solve_opts *opts, double *d, double *y,
                    double *x, arh_solve_stats *stats, arh_hmat_info *info);
 
+/* ------------------------------------------------------------------ serving (continuous batching) */
+/* New (the reference has no serving path; its callers loop over DiffBEM.band_value one call at a
+ * time, acoustics_pipeline.py:28-33). A server owns `workers` device streams (one arh_ctx each).
+ * Requests -- one band solve each (arh_band_solve's unit) -- are admitted in arrival order as
+ * soon as a worker is free and their H-matrix footprint fits the HBM budget; admitted requests run
+ * concurrently on the device. Every request is computed by the same deterministic kernels whatever
+ * else is in flight, so its result is the same alone and in the mix. */
+typedef struct {
+    int workers;        /* concurrent requests (device streams) */
+    int64_t hbm_budget; /* bytes the in-flight requests may hold (0 = 90% of free HBM at creation) */
+    int device;
+} arh_server_opts;
+
+typedef struct {
+    int status;            /* 0 queued, 1 running, 2 done, < 0 the request's ARH_E* code */
+    int nrhs;
+    double t_submit, t_start, t_done; /* seconds on the server's steady clock */
+    int64_t footprint;     /* admitted HBM bytes (estimate) */
+    int iters;             /* BiCGSTAB iterations (max over rhs) */
+    double max_true_res;   /* FP64 true residual (max over rhs) */
+} arh_request_info;
+
+int arh_server_create(const arh_server_opts *opts, arh_server **out);
+/* Register a mesh (original element order; the server partitions and reorders it). Host inputs. */
+int arh_server_add_mesh(arh_server *s, const double *Ps, int V, const int *Es, int N, int leaf_size,
+                        double radius_factor, int *mesh_id);
+/* Queue one band solve; vecs (3*nrhs) and L (3*LL) are copied. Returns immediately. */
+int arh_server_submit(arh_server *s, int mesh_id, double k, double aca_tol, int storage, int kind, int nrhs,
+                      const double *vecs, const double *L, int LL, double tol, int64_t *request_id);
+/* Block until the request is done; d (host, nrhs) receives the diffusion coefficients. */
+int arh_server_wait(arh_server *s, int64_t request_id, double *d, arh_request_info *info);
+int arh_server_poll(arh_server *s, int64_t request_id, arh_request_info *info);
+void arh_server_destroy(arh_server *s);
+
 #ifdef __cplusplus
 }
 #endif
