@@ -156,6 +156,15 @@ struct DenseTask {
     int leaf, c0, nc, pad;
 };
 
+// MFMA product column group: up to 4 consecutive complex columns of one operand segment of a row
+// leaf. kind 0: dense panel columns (A at panel + a_off, rhs rows b_row.. of the packed input);
+// kind 1: U columns of a low-rank block (A at U + a_off, rhs rows b_row.. of the intermediates).
+struct ColGroup {
+    int64_t a_off;
+    int b_row;
+    short ncols, kind;
+};
+
 // Low-rank block descriptor (device). Offsets are in complex elements.
 struct LRBlock {
     int rs, re, cs, ce;
@@ -222,9 +231,14 @@ struct arh_hmat_s {
     std::vector<int> active_lr;  // lr blocks with rank > 0
     arh::DBuf d_active_lr;
     arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
-    // MFMA multi-rhs product: packed FP32 rhs, FP32 intermediates, phase-1 work units (block, row tile)
-    arh::DBuf xf32, t32, d_p1units;
+    // MFMA multi-rhs product: packed FP32 rhs, FP32 intermediates, phase-1 work units (block, 32-row
+    // tile; blocks with many columns first, one workgroup each), per-leaf column groups
+    arh::DBuf xf32, t32, d_p1units, d_cgroups, d_cg_ptr;
     std::vector<int2> p1units;
+    int p1big = 0;
+    std::vector<ColGroup> cgroups;
+    std::vector<int> cg_ptr;
+    bool mfma_ready = false;
     // FP64 product: phase-1 units (block, 4-row group), phase-2 units per leaf, per-unit partials
     std::vector<MvUnit> mv_units;
     std::vector<int> mv_uptr;

@@ -207,6 +207,20 @@ __global__ __launch_bounds__(256) void k_pack(const LRBlock *__restrict__ blocks
     }
 }
 
+// FP32 copy of V, transposed per block to column-major (n x r, V32[v_off + c r + p]): the MFMA
+// product reads 16 consecutive rank rows of one column.
+__global__ __launch_bounds__(256) void k_v_to_c64_cm(const LRBlock *__restrict__ blocks, const int *__restrict__ active,
+                                                     const double2 *__restrict__ V, float2 *__restrict__ V32) {
+    const LRBlock B = blocks[active[blockIdx.x]];
+    const int n = B.ce - B.cs, r = B.rank;
+    const int64_t tot = (int64_t)r * n;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int c = (int)(e / r), p = (int)(e - (int64_t)c * r);
+        const double2 v = V[B.v_off + (int64_t)p * n + c];
+        V32[B.v_off + e] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
 __global__ void k_to_c64(const double2 *__restrict__ a, float2 *__restrict__ b, int64_t n) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         double2 v = a[e];
@@ -329,7 +343,12 @@ void make_fp32_copy(arh_hmat *h) {
     };
     conv(h->dense, h->dense32, g->dense_entries);
     conv(h->U, h->U32, h->u_total);
-    conv(h->V, h->V32, h->v_total);
+    h->V32.alloc(sizeof(float2) * std::max<int64_t>(1, h->v_total));
+    if (!h->active_lr.empty()) {
+        hipLaunchKernelGGL(k_v_to_c64_cm, dim3((unsigned)h->active_lr.size()), dim3(256), 0, s, h->blocks.as<LRBlock>(),
+                           h->d_active_lr.as<int>(), h->V.as<double2>(), h->V32.as<float2>());
+        ARH_HIP(hipGetLastError());
+    }
     ARH_HIP(hipStreamSynchronize(s));
 }
 

@@ -35,22 +35,27 @@ struct CT<float2> {
     }
 };
 
+// V entry (p, c) of a block: row-major (FP64 copy) or column-major (FP32 copy)
+template <class S>
+__device__ __forceinline__ const S *v_at(const S *V, const LRBlock &B, int p, int c, bool vcm) {
+    return V + B.v_off + (vcm ? (int64_t)c * B.rank + p : (int64_t)p * (B.ce - B.cs) + c);
+}
+
 // ---------------------------------------------------------------- phase 1 (forward): t = V x
 template <class S>
 __global__ __launch_bounds__(kMvThreads) void k_lr_t(const LRBlock *__restrict__ blocks, const int *__restrict__ active,
-                                                     const S *__restrict__ V, const double2 *__restrict__ x, int N,
-                                                     int R, double2 *__restrict__ t) {
+                                                     const S *__restrict__ V, int vcm, const double2 *__restrict__ x,
+                                                     int N, int R, double2 *__restrict__ t) {
     const LRBlock B = blocks[active[blockIdx.x]];
     const int n = B.ce - B.cs, r = B.rank;
     const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
     for (int p = w; p < r; p += kMvWaves) {
-        const S *vr = V + B.v_off + (int64_t)p * n;
         for (int j0 = 0; j0 < R; j0 += RC) {
             double2 acc[RC];
 #pragma unroll
             for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
             for (int c = lane; c < n; c += ARH_WAVE) {
-                double2 v = CT<S>::ld(vr + c);
+                double2 v = CT<S>::ld(v_at(V, B, p, c, vcm));
 #pragma unroll
                 for (int q = 0; q < RC; q++)
                     if (j0 + q < R) cfma(acc[q], v, x[(int64_t)(j0 + q) * N + B.cs + c]);
@@ -172,8 +177,9 @@ __global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ lea
                                                      const int64_t *__restrict__ panel_off, const S *__restrict__ panel,
                                                      const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
                                                      const LRBlock *__restrict__ blocks, const S *__restrict__ V,
-                                                     const double2 *__restrict__ t, const double2 *__restrict__ x,
-                                                     int N, int R, double2 *__restrict__ y) {
+                                                     int vcm, const double2 *__restrict__ t,
+                                                     const double2 *__restrict__ x, int N, int R,
+                                                     double2 *__restrict__ y) {
     __shared__ double2 red[kMvWaves * ARH_WAVE * RC];
     const int l = blockIdx.x;
     const int cs = leaf_start[l], n = leaf_start[l + 1] - cs;
@@ -202,10 +208,9 @@ __global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ lea
                 }
                 for (int e = b0; e < b1; e++) {
                     const LRBlock &B = blocks[lr_blk[e]];
-                    const int r = B.rank, nb = B.ce - B.cs;
-                    const S *vc = V + B.v_off + (cs - B.cs) + c;
+                    const int r = B.rank;
                     for (int p = w; p < r; p += kMvWaves) {
-                        double2 a = CT<S>::ld(vc + (int64_t)p * nb);
+                        double2 a = CT<S>::ld(v_at(V, B, p, (cs - B.cs) + c, vcm));
 #pragma unroll
                         for (int q = 0; q < RC; q++)
                             if (j0 + q < R) cfma(acc[q], a, t[(B.t_off + p) * R + j0 + q]);
@@ -262,7 +267,7 @@ static void apply_t(arh_hmat *h, int transpose, int R, const S *panel, const S *
         timed(fp32 ? PROF_LR32 : PROF_LR64, es * lr_entries + vs * (N + h->t_total), 8.0 * R * lr_entries, [&] {
             if (!transpose)
                 hipLaunchKernelGGL(k_lr_t<S>, dim3(nact), dim3(kMvThreads), 0, s, h->blocks.as<LRBlock>(),
-                                   h->d_active_lr.as<int>(), V, x, N, R, t);
+                                   h->d_active_lr.as<int>(), V, (int)fp32, x, N, R, t);
             else
                 hipLaunchKernelGGL(k_lr_s<S>, dim3(nact), dim3(kMvThreads), 0, s, h->blocks.as<LRBlock>(),
                                    h->d_active_lr.as<int>(), g->d_leaf_start.as<int>(), U, x, N, R, t);
@@ -277,7 +282,7 @@ static void apply_t(arh_hmat *h, int transpose, int R, const S *panel, const S *
             hipLaunchKernelGGL(k_cols<S>, dim3(g->nleaf), dim3(kMvThreads), 0, s, g->d_leaf_start.as<int>(),
                                g->d_dT_ptr.as<int>(), g->d_dT_blk.as<int>(), g->d_dblk_info.as<int>(),
                                g->d_panel_off.as<int64_t>(), panel, g->d_lrC_ptr.as<int>(), g->d_lrC_blk.as<int>(),
-                               h->blocks.as<LRBlock>(), V, t, x, N, R, y);
+                               h->blocks.as<LRBlock>(), V, (int)fp32, t, x, N, R, y);
     });
     ARH_HIP(hipGetLastError());
 }
