@@ -156,13 +156,26 @@ struct DenseTask {
     int leaf, c0, nc, pad;
 };
 
-// MFMA product column group: up to 4 consecutive complex columns of one operand segment of a row
-// leaf. kind 0: dense panel columns (A at panel + a_off, rhs rows b_row.. of the packed input);
-// kind 1: U columns of a low-rank block (A at U + a_off, rhs rows b_row.. of the intermediates).
-struct ColGroup {
-    int64_t a_off;
-    int b_row;
-    short ncols, kind;
+// MFMA product: the low-rank V factors of all blocks with one column cluster, stacked (column-
+// major: Vs[v_off + c rtot + p], p < rtot = sum of the blocks' ranks).
+struct VStack {
+    int cs, n, rtot, pad;
+    int64_t v_off;  // offset in the stacked FP32 V
+    int64_t t_row;  // row of the stack's first intermediate in the packed [x; t] buffer
+};
+
+// MFMA product: one row leaf's fused FP32 panel (dense columns, then the U columns of the low-rank
+// blocks covering it, then zero columns up to a multiple of 4), m x ncol column-major at `off`;
+// the packed [x; t] row feeding panel column c is src[col0 + c].
+struct LeafPanel {
+    int64_t off;
+    int col0, ngroups;  // ngroups = ncol / 4
+};
+
+// MFMA product row task: column groups [g0, g1) of row leaf `leaf`; slot >= 0 names the task's
+// partial tile when the leaf is split over several tasks, -1 when the task writes y itself.
+struct RowTask {
+    int leaf, g0, g1, slot;
 };
 
 // Low-rank block descriptor (device). Offsets are in complex elements.
@@ -231,14 +244,21 @@ struct arh_hmat_s {
     std::vector<int> active_lr;  // lr blocks with rank > 0
     arh::DBuf d_active_lr;
     arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
-    // MFMA multi-rhs product: packed FP32 rhs, FP32 intermediates, phase-1 work units (block, 32-row
-    // tile; blocks with many columns first, one workgroup each), per-leaf column groups
-    arh::DBuf xf32, t32, d_p1units, d_cgroups, d_cg_ptr;
+    // MFMA multi-rhs product (FP32 copy in its own layout): fused row-leaf panels, V stacks, the
+    // packed rhs + intermediates buffer ([x; t] rows of RP complex), phase-1 units (stack, 32-row
+    // tile; stacks with many columns first, one workgroup each), row tasks (largest first), split
+    // leaves (leaf, first partial slot, task count) and their partial tiles
+    arh::DBuf panel32, vs32, xt32, part32;
+    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit;
+    std::vector<VStack> stacks;
     std::vector<int2> p1units;
     int p1big = 0;
-    std::vector<ColGroup> cgroups;
-    std::vector<int> cg_ptr;
-    bool mfma_ready = false;
+    std::vector<LeafPanel> lpanel;
+    std::vector<RowTask> rtasks;
+    std::vector<int4> rsplit;
+    int nslots = 0;
+    int64_t panel32_total = 0, vs32_total = 0;
+    bool generic32 = false;  // dense32 / U32 / V32 (the non-MFMA FP32 product) built
     // FP64 product: phase-1 units (block, 4-row group), phase-2 units per leaf, per-unit partials
     std::vector<MvUnit> mv_units;
     std::vector<int> mv_uptr;
@@ -254,10 +274,12 @@ void geom_build_tri(arh_geom *g);
 void assemble_dense(arh_hmat *h);
 void assemble_aca(arh_hmat *h);
 void make_fp32_copy(arh_hmat *h);
+void make_generic_fp32(arh_hmat *h);
 // matvec.hip
 void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s);
 // matvec_mfma.hip
 void hmat_apply_mfma(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+void mfma_prepare(arh_hmat *h);
 // matvec64.hip
 void hmat_apply64(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
 int mfma_max_leaf();

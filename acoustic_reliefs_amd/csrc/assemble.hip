@@ -330,7 +330,10 @@ void assemble_aca(arh_hmat *h) {
     ARH_HIP(hipStreamSynchronize(s));
 }
 
-void make_fp32_copy(arh_hmat *h) {
+// FP32 copy for the generic (non-MFMA) product: dense panels and U elementwise, V column-major
+// per block. Built on first use.
+void make_generic_fp32(arh_hmat *h) {
+    if (h->generic32) return;
     arh_geom *g = h->g;
     hipStream_t s = g->ctx->stream;
     auto conv = [&](DBuf &src, DBuf &dst, int64_t n) {
@@ -350,6 +353,15 @@ void make_fp32_copy(arh_hmat *h) {
         ARH_HIP(hipGetLastError());
     }
     ARH_HIP(hipStreamSynchronize(s));
+    h->generic32 = true;
+}
+
+// FP32 copy at assembly: the MFMA layout when the leaves fit it, else the generic one.
+void make_fp32_copy(arh_hmat *h) {
+    if (h->g->max_leaf <= mfma_max_leaf())
+        mfma_prepare(h);
+    else
+        make_generic_fp32(h);
 }
 
 }  // namespace arh

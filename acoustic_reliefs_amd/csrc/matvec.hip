@@ -295,6 +295,7 @@ void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const doubl
             hmat_apply_mfma(h, nrhs, x, y, s);
             return;
         }
+        make_generic_fp32(h);
         apply_t<float2>(h, transpose, nrhs, h->dense32.as<float2>(), h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
     } else {
         if (!transpose) {
