@@ -81,10 +81,16 @@ struct Profiler {
         hipEvent_t a, b;
         int kind;
         double bytes, flops;
+        int slot = -1;  // gated launch: index into slot_ok (see below), else -1
     };
     bool on = false;
     std::vector<hipEvent_t> pool;
     std::vector<Entry> pending;
+    // Gated launches (skipped on the device when the solver's active count is 0): a launcher
+    // tags its entries with cur_slot; before flushing, the solver sets slot_ok[slot] to whether
+    // that slot's launches ran, and launches that did not are left out of the totals.
+    int cur_slot = -1;
+    std::vector<char> slot_ok;
     arh_profile acc{};
     hipEvent_t get() {
         if (pool.empty()) {
@@ -114,10 +120,13 @@ struct Profiler {
                 pending[keep++] = e;
                 continue;
             }
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e.a, e.b);
-            arh_kernel_prof &s = slot(e.kind);
-            s.launches++, s.ms += ms, s.bytes += e.bytes, s.flops += e.flops;
+            const bool ran = e.slot < 0 || e.slot >= (int)slot_ok.size() || slot_ok[e.slot];
+            if (ran) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e.a, e.b);
+                arh_kernel_prof &s = slot(e.kind);
+                s.launches++, s.ms += ms, s.bytes += e.bytes, s.flops += e.flops;
+            }
             pool.push_back(e.a), pool.push_back(e.b);
         }
         pending.resize(keep);
@@ -276,9 +285,14 @@ void assemble_aca(arh_hmat *h);
 void make_fp32_copy(arh_hmat *h);
 void make_generic_fp32(arh_hmat *h);
 // matvec.hip
-void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+// y = (M - I/2) x (or its transpose) for nrhs columns. cols (device, ncols entries) restricts the
+// MFMA product to those columns (other products compute all); gate (device int) lets the MFMA
+// product skip itself when 0. Both are optional.
+void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s,
+                const int *cols = nullptr, int ncols = 0, const int *gate = nullptr);
 // matvec_mfma.hip
-void hmat_apply_mfma(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
+void hmat_apply_mfma(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s, const int *cols = nullptr,
+                     int ncols = 0, const int *gate = nullptr);
 void mfma_prepare(arh_hmat *h);
 // matvec64.hip
 void hmat_apply64(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);

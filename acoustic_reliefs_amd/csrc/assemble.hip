@@ -10,6 +10,7 @@ namespace arh {
 constexpr int kAcaThreads = 256;
 constexpr int kAcaWaves = kAcaThreads / ARH_WAVE;
 constexpr int kMaxRankLds = 128;  // max ACA rank kept in LDS coefficient arrays
+constexpr int kAcaBigDim = 1024;  // blocks with m + n at least this get a workgroup, smaller ones a wavefront
 
 // ------------------------------------------------------------------ dense panels
 // One thread per entry of a (row leaf, column chunk) task; entries of a column are contiguous,
@@ -34,59 +35,73 @@ __global__ __launch_bounds__(256) void k_dense(const DenseTask *__restrict__ tas
     out[panel_off[t.leaf] + (int64_t)c * m + i] = v;
 }
 
-// ------------------------------------------------------------------ block reductions (256 threads)
+// ------------------------------------------------------------------ ACA cooperation groups
 struct AcaShared {
     double2 coef[kMaxRankLds];
     double du[kMaxRankLds], dv[kMaxRankLds];
     double rv[kAcaWaves];
     int ri[kAcaWaves];
-    double bcast;
-    int ibcast;
 };
 
-__device__ __forceinline__ double block_sum(double v, AcaShared &sh) {
-    v = wave_sum(v);
-    const int w = threadIdx.x / ARH_WAVE, lane = threadIdx.x % ARH_WAVE;
-    __syncthreads();
-    if (lane == 0) sh.rv[w] = v;
-    __syncthreads();
-    double s = sh.rv[0];
+// The threads working on one ACA block: the whole 256-thread workgroup (WAVE false) or a single
+// wavefront (WAVE true, no workgroup barrier: a wavefront's memory operations complete in order,
+// so a fence at wavefront scope, which also stops compiler reordering, is its barrier).
+template <bool WAVE>
+struct AcaGroup {
+    int tid, nthr, w, nw, lane;
+    AcaShared &sh;
+    __device__ __forceinline__ void sync() const {
+        if (WAVE) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            __syncthreads();
+        }
+    }
+    // fixed-order sums: all threads get the same bits
+    __device__ __forceinline__ double sum(double v) const {
+        v = wave_sum(v);
+        if (WAVE) return v;
+        sync();
+        if (lane == 0) sh.rv[w] = v;
+        sync();
+        double s = sh.rv[0];
 #pragma unroll
-    for (int q = 1; q < kAcaWaves; q++) s += sh.rv[q];
-    return s;
-}
-
-__device__ __forceinline__ void block_argmax(double &v, int &i, AcaShared &sh) {
-    wave_argmax(v, i);
-    const int w = threadIdx.x / ARH_WAVE, lane = threadIdx.x % ARH_WAVE;
-    __syncthreads();
-    if (lane == 0) sh.rv[w] = v, sh.ri[w] = i;
-    __syncthreads();
-    double bv = sh.rv[0];
-    int bi = sh.ri[0];
+        for (int q = 1; q < kAcaWaves; q++) s += sh.rv[q];
+        return s;
+    }
+    // (value, index) max with ties to the lower index
+    __device__ __forceinline__ void argmax(double &v, int &i) const {
+        wave_argmax(v, i);
+        if (WAVE) return;
+        sync();
+        if (lane == 0) sh.rv[w] = v, sh.ri[w] = i;
+        sync();
+        double bv = sh.rv[0];
+        int bi = sh.ri[0];
 #pragma unroll
-    for (int q = 1; q < kAcaWaves; q++)
-        if (sh.rv[q] > bv || (sh.rv[q] == bv && sh.ri[q] < bi)) bv = sh.rv[q], bi = sh.ri[q];
-    v = bv;
-    i = bi;
-}
+        for (int q = 1; q < kAcaWaves; q++)
+            if (sh.rv[q] > bv || (sh.rv[q] == bv && sh.ri[q] < bi)) bv = sh.rv[q], bi = sh.ri[q];
+        v = bv;
+        i = bi;
+    }
+};
 
-// ------------------------------------------------------------------ ACA, one workgroup per block
+// ------------------------------------------------------------------ ACA of one block
 // compute_ACA_block (aca.cpp:38-82): start at row 0; residual row, first-max pivot column,
 // normalised row (unnormalised if the pivot is 0), residual column, non-conjugated norm
 // estimate, stop when sqrt(|u|^2|v|^2 / err) <= eps discarding the current cross, next row =
 // first max of |u| over unused rows; rank limit min(max_rank, m-1, n-1).
 // Crosses whose u or v is exactly zero add nothing to any later quantity, so they are counted
 // (rank_ref) but not stored (rank).
-__global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ blocks, const int *__restrict__ order,
-                                                     const Tri *__restrict__ tri, double k, double eps,
-                                                     double2 *__restrict__ Us, double2 *__restrict__ Vs,
-                                                     unsigned char *__restrict__ flags) {
-    __shared__ AcaShared sh;
-    const int bid = order[blockIdx.x];
-    LRBlock B = blocks[bid];
+template <bool WAVE>
+__device__ __forceinline__ void aca_block(const AcaGroup<WAVE> &G, LRBlock *__restrict__ blocks, int bid,
+                                          const Tri *__restrict__ tri, double k, double eps, double2 *__restrict__ Us,
+                                          double2 *__restrict__ Vs, unsigned char *__restrict__ flags) {
+    AcaShared &sh = G.sh;
+    const LRBlock B = blocks[bid];
     const int rs = B.rs, cs = B.cs, m = B.re - B.rs, n = B.ce - B.cs, lim = B.lim;
-    const int tid = threadIdx.x, lane = tid % ARH_WAVE, w = tid / ARH_WAVE;
+    const int tid = G.tid, nthr = G.nthr, lane = G.lane;
     if (lim <= 0) {
         if (tid == 0) blocks[bid].rank = 0, blocks[bid].rank_ref = 0;
         return;
@@ -94,20 +109,20 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ block
     double2 *U = Us + B.us_off;
     double2 *V = Vs + B.vs_off;
     unsigned char *used = flags + B.flag_off;
-    for (int q = tid; q < m; q += kAcaThreads) used[q] = (q == 0) ? 1 : 0;
+    for (int q = tid; q < m; q += nthr) used[q] = (q == 0) ? 1 : 0;
     int i = 0, it = 0, sr = 0;
     double err = 0.0;
-    __syncthreads();
+    G.sync();
     while (it < lim) {
         // coefficients U(i, p), p < sr
-        for (int p = tid; p < sr; p += kAcaThreads) sh.coef[p] = U[(int64_t)p * m + i];
-        __syncthreads();
+        for (int p = tid; p < sr; p += nthr) sh.coef[p] = U[(int64_t)p * m + i];
+        G.sync();
         // residual row into V row `sr`, with the first-max pivot search
         const Tri ti = tri[rs + i];
         double2 *vrow = V + (int64_t)sr * n;
         double best = -1.0;
         int bi = 0x7fffffff;
-        for (int c = tid; c < n; c += kAcaThreads) {
+        for (int c = tid; c < n; c += nthr) {
             double2 v = bem_entry(k, ti.gx, ti.gy, ti.gz, tri[cs + c]);
             for (int p = 0; p < sr; p++) {
                 double2 vp = V[(int64_t)p * n + c];
@@ -119,27 +134,29 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ block
             double a = cnorm2(v);
             if (a > best) best = a, bi = c;
         }
-        block_argmax(best, bi, sh);
+        G.argmax(best, bi);
         const int j = (best < 0.0) ? 0 : bi;
-        __syncthreads();
+        G.sync();
         const double2 piv = vrow[j];
+        // every thread holds the pivot before its owner normalises it in place
+        G.sync();
         const bool nz = (piv.x != 0.0 || piv.y != 0.0);
         double vn = 0.0;
-        for (int c = tid; c < n; c += kAcaThreads) {
+        for (int c = tid; c < n; c += nthr) {
             double2 v = vrow[c];
             if (nz) v = cdiv(v, piv);
             vrow[c] = v;
             vn += cnorm2(v);
         }
-        vn = block_sum(vn, sh);
-        for (int p = tid; p < sr; p += kAcaThreads) sh.coef[p] = V[(int64_t)p * n + j];
-        __syncthreads();
+        vn = G.sum(vn);
+        for (int p = tid; p < sr; p += nthr) sh.coef[p] = V[(int64_t)p * n + j];
+        G.sync();
         // residual column into U column `sr`, with the masked first-max search for the next row
         const Tri tj = tri[cs + j];
         double2 *ucol = U + (int64_t)sr * m;
         double un = 0.0, bu = -1.0;
         int bq = 0x7fffffff;
-        for (int q = tid; q < m; q += kAcaThreads) {
+        for (int q = tid; q < m; q += nthr) {
             const Tri tq = tri[rs + q];
             double2 u = bem_entry(k, tq.gx, tq.gy, tq.gz, tj);
             for (int p = 0; p < sr; p++) {
@@ -153,11 +170,11 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ block
             un += a;
             if (!used[q] && a > bu) bu = a, bq = q;
         }
-        un = block_sum(un, sh);
-        block_argmax(bu, bq, sh);
-        __syncthreads();
-        // |u_p^T u| and |v v_p^T| for stored crosses p < sr (aca.cpp:64-69): wave w takes p = w mod 4
-        for (int p = w; p < sr; p += kAcaWaves) {
+        un = G.sum(un);
+        G.argmax(bu, bq);
+        G.sync();
+        // |u_p^T u| and |v v_p^T| for stored crosses p < sr (aca.cpp:64-69): wave w takes p = w mod nw
+        for (int p = G.w; p < sr; p += G.nw) {
             double2 su = make_double2(0.0, 0.0), sv = make_double2(0.0, 0.0);
             const double2 *up = U + (int64_t)p * m;
             const double2 *vp = V + (int64_t)p * n;
@@ -167,19 +184,39 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ block
             sv = wave_sum2(sv);
             if (lane == 0) sh.du[p] = cabs_(su), sh.dv[p] = cabs_(sv);
         }
-        __syncthreads();
+        G.sync();
         double dots = 0.0;
         for (int p = 0; p < sr; p++) dots += sh.du[p] * sh.dv[p];
         err += un * vn + 2.0 * dots;
         if (sqrt(un * vn / err) <= eps) break;
         if (un * vn > 0.0) sr++;
         i = (bu < 0.0) ? 0 : bq;
-        __syncthreads();
+        G.sync();
         if (tid == 0) used[i] = 1;
         it++;
-        __syncthreads();
+        G.sync();
     }
     if (tid == 0) blocks[bid].rank = sr, blocks[bid].rank_ref = it;
+}
+
+// Workgroups [0, nbig) each take one big block (order[0..nbig)); the others take four small
+// blocks, one per wavefront.
+__global__ __launch_bounds__(kAcaThreads) void k_aca(LRBlock *__restrict__ blocks, const int *__restrict__ order,
+                                                     int nbig, int na, const Tri *__restrict__ tri, double k,
+                                                     double eps, double2 *__restrict__ Us, double2 *__restrict__ Vs,
+                                                     unsigned char *__restrict__ flags) {
+    __shared__ AcaShared sh[kAcaWaves];
+    const int lane = threadIdx.x % ARH_WAVE;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / ARH_WAVE));
+    if ((int)blockIdx.x < nbig) {
+        const AcaGroup<false> G{(int)threadIdx.x, kAcaThreads, w, kAcaWaves, lane, sh[0]};
+        aca_block<false>(G, blocks, order[blockIdx.x], tri, k, eps, Us, Vs, flags);
+        return;
+    }
+    const int u = nbig + ((int)blockIdx.x - nbig) * kAcaWaves + w;
+    if (u >= na) return;
+    const AcaGroup<true> G{lane, ARH_WAVE, 0, 1, lane, sh[w]};
+    aca_block<true>(G, blocks, order[u], tri, k, eps, Us, Vs, flags);
 }
 
 // ------------------------------------------------------------------ compaction
@@ -298,11 +335,23 @@ void assemble_aca(arh_hmat *h) {
         double wy = (double)Y.lim * Y.lim * (Y.re - Y.rs + Y.ce - Y.cs) + (Y.re - Y.rs + Y.ce - Y.cs);
         return wx > wy;
     });
+    // blocks with m + n >= kAcaBigDim get a workgroup each (they lead the order, largest first);
+    // the rest one wavefront each
+    std::stable_partition(order.begin(), order.end(), [&](int x) {
+        const LRBlock &X = h->hblocks[x];
+        return X.re - X.rs + X.ce - X.cs >= kAcaBigDim;
+    });
+    int nbig = 0;
+    while (nbig < na && h->hblocks[order[nbig]].re - h->hblocks[order[nbig]].rs + h->hblocks[order[nbig]].ce -
+                                h->hblocks[order[nbig]].cs >=
+                            kAcaBigDim)
+        nbig++;
     upload(h->blocks, h->hblocks, s);
     DBuf d_order;
     upload(d_order, order, s);
-    hipLaunchKernelGGL(k_aca, dim3(na), dim3(kAcaThreads), 0, s, h->blocks.as<LRBlock>(), d_order.as<int>(),
-                       g->tri.as<Tri>(), h->k, h->aca_tol, g->aca_U.as<double2>(), g->aca_V.as<double2>(),
+    const int grid = nbig + (na - nbig + kAcaWaves - 1) / kAcaWaves;
+    hipLaunchKernelGGL(k_aca, dim3(grid), dim3(kAcaThreads), 0, s, h->blocks.as<LRBlock>(), d_order.as<int>(), nbig,
+                       na, g->tri.as<Tri>(), h->k, h->aca_tol, g->aca_U.as<double2>(), g->aca_V.as<double2>(),
                        g->aca_flags.as<unsigned char>());
     ARH_HIP(hipGetLastError());
     ARH_HIP(hipMemcpyAsync(h->hblocks.data(), h->blocks.p, sizeof(LRBlock) * na, hipMemcpyDeviceToHost, s));

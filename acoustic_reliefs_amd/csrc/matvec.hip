@@ -287,12 +287,13 @@ static void apply_t(arh_hmat *h, int transpose, int R, const S *panel, const S *
     ARH_HIP(hipGetLastError());
 }
 
-void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s) {
+void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s,
+                const int *cols, int ncols, const int *gate) {
     ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
     if (precision == ARH_MIXED) {
         ARH_CHECK(h->storage == ARH_MIXED, ARH_ESTATE, "FP32 product requested but the H-matrix has no FP32 copy");
         if (!transpose && nrhs >= 4 && h->g->max_leaf <= mfma_max_leaf()) {
-            hmat_apply_mfma(h, nrhs, x, y, s);
+            hmat_apply_mfma(h, nrhs, x, y, s, cols, ncols, gate);
             return;
         }
         make_generic_fp32(h);

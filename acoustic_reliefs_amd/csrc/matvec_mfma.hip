@@ -57,14 +57,22 @@ constexpr int kPartRows = 64;     // rows of a partial tile (the largest leaf)
 constexpr int kTaskSlots = 1024;  // row tasks aimed for: a few per compute unit
 constexpr int kMaxRC = 40;        // right-hand sides per pass (RP <= 40)
 
-// xt[i * RP + j] = (float) x[j * N + i] for j < R, 0 for R <= j < RP (rows i < N)
-__global__ void k_pack_xf(const double2 *__restrict__ x, int N, int R, int RP, float2 *__restrict__ xf) {
+// Right-hand side j of a pass is column cols[j0 + j] of x / y (j0 + j without a list).
+__device__ __forceinline__ int64_t col_of(const int *cols, int j0, int j) { return cols ? cols[j0 + j] : j0 + j; }
+
+// A pass is skipped when the gate (the solver's count of active right-hand sides) is 0.
+__device__ __forceinline__ bool gated(const int *gate) { return gate && *gate == 0; }
+
+// xt[i * RP + j] = (float) x[col j][i] for j < R, 0 for R <= j < RP (rows i < N)
+__global__ void k_pack_xf(const double2 *__restrict__ x, int N, int R, int RP, float2 *__restrict__ xf,
+                          const int *__restrict__ cols, int j0, const int *__restrict__ gate) {
+    if (gated(gate)) return;
     const int64_t total = (int64_t)N * RP;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e / RP), j = (int)(e % RP);
         float2 v = make_float2(0.f, 0.f);
         if (j < R) {
-            double2 d = x[(int64_t)j * N + i];
+            double2 d = x[col_of(cols, j0, j) * N + i];
             v = make_float2((float)d.x, (float)d.y);
         }
         xf[e] = v;
@@ -253,8 +261,9 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const float2 *
 template <int NT>
 __global__ __launch_bounds__(64 * kWaves) void k_lr_mfma(const VStack *__restrict__ stacks, const int2 *__restrict__ units,
                                                  int nbig, int nunits, const float2 *__restrict__ V,
-                                                 float2 *__restrict__ xt) {
+                                                 float2 *__restrict__ xt, const int *__restrict__ gate) {
     __shared__ float red[(kWaves / 2) * 2 * NT * 256];
+    if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const bool big = (int)blockIdx.x < nbig;
     const int u = big ? (int)blockIdx.x : nbig + ((int)blockIdx.x - nbig) * kWaves + w;
@@ -277,8 +286,8 @@ template <int NT, int NRT>
 __device__ __forceinline__ void rows_run(const RowTask &T, int rs, int m, const LeafPanel &L,
                                          const int *__restrict__ psrc, const float2 *__restrict__ panel,
                                          const float2 *__restrict__ xt, const double *__restrict__ x, int N, int R,
-                                         double *__restrict__ y, float *__restrict__ part, float *red, int w,
-                                         int lane) {
+                                         double *__restrict__ y, const int *__restrict__ cols, int j0,
+                                         float *__restrict__ part, float *red, int w, int lane) {
     constexpr int RP = 8 * NT;
     const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
     f4 acc[NRT][NT];
@@ -303,7 +312,7 @@ __device__ __forceinline__ void rows_run(const RowTask &T, int rs, int m, const 
     } else {
         for (int e = threadIdx.x; e < 2 * m * R; e += blockDim.x) {
             const int j = e / (2 * m), q = e - j * 2 * m, row = q >> 1, p = q & 1;
-            const int64_t idx = ((int64_t)j * N + rs + row) * 2 + p;
+            const int64_t idx = (col_of(cols, j0, j) * N + rs + row) * 2 + p;
             y[idx] = -0.5 * x[idx] + (double)red[row * 2 * RP + 2 * j + p];
         }
     }
@@ -314,8 +323,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(const RowTask *__rest
                                                    const LeafPanel *__restrict__ lpanel, const int *__restrict__ psrc,
                                                    const float2 *__restrict__ panel, const float2 *__restrict__ xt,
                                                    const double *__restrict__ x, int N, int R, double *__restrict__ y,
+                                                   const int *__restrict__ cols, int j0, const int *__restrict__ gate,
                                                    float *__restrict__ part) {
     __shared__ float red[(kWaves / 2) * MAXRT * NT * 256];
+    if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const RowTask T = tasks[blockIdx.x];
     if (T.leaf < 0) return;  // padding task
@@ -323,14 +334,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(const RowTask *__rest
     const LeafPanel L = lpanel[T.leaf];
     const int nrt = (m + 15) >> 4;
     if (nrt <= 1)
-        rows_run<NT, 1>(T, rs, m, L, psrc, panel, xt, x, N, R, y, part, red, w, lane);
+        rows_run<NT, 1>(T, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red, w, lane);
     else if (nrt == 2 || MAXRT == 2)
-        rows_run<NT, 2>(T, rs, m, L, psrc, panel, xt, x, N, R, y, part, red, w, lane);
+        rows_run<NT, 2>(T, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red, w, lane);
     else if constexpr (MAXRT >= 4) {
         if (nrt == 3)
-            rows_run<NT, 3>(T, rs, m, L, psrc, panel, xt, x, N, R, y, part, red, w, lane);
+            rows_run<NT, 3>(T, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red, w, lane);
         else
-            rows_run<NT, 4>(T, rs, m, L, psrc, panel, xt, x, N, R, y, part, red, w, lane);
+            rows_run<NT, 4>(T, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red, w, lane);
     }
 }
 
@@ -339,9 +350,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(const RowTask *__rest
 template <int NT>
 __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split, const int *__restrict__ leaf_start,
                                                   const float *__restrict__ part, const double *__restrict__ x, int N,
-                                                  int R, double *__restrict__ y) {
+                                                  int R, double *__restrict__ y, const int *__restrict__ cols, int j0,
+                                                  const int *__restrict__ gate) {
     constexpr int W = 16 * NT;
     __shared__ float tile[kPartRows * W];
+    if (gated(gate)) return;
     const int4 S = split[blockIdx.x];
     const int rs = leaf_start[S.x], m = leaf_start[S.x + 1] - rs;
     const float *src = part + (int64_t)S.y * kPartRows * W;
@@ -353,7 +366,7 @@ __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split
     __syncthreads();
     for (int e = threadIdx.x; e < 2 * m * R; e += blockDim.x) {
         const int j = e / (2 * m), q = e - j * 2 * m, row = q >> 1, p = q & 1;
-        const int64_t idx = ((int64_t)j * N + rs + row) * 2 + p;
+        const int64_t idx = (col_of(cols, j0, j) * N + rs + row) * 2 + p;
         y[idx] = -0.5 * x[idx] + (double)tile[row * W + 2 * j + p];
     }
 }
@@ -536,7 +549,8 @@ void mfma_prepare(arh_hmat *h) {
 }
 
 template <int NT>
-static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, hipStream_t s, int maxrt) {
+static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, const int *cols, int j0, const int *gate,
+                      hipStream_t s, int maxrt) {
     arh_geom *g = h->g;
     Profiler &pf = g->ctx->prof;
     const int N = g->N;
@@ -545,13 +559,13 @@ static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, hipStrea
     if (h->nslots) h->part32.alloc(sizeof(float) * (size_t)h->nslots * kPartRows * 2 * RP);
     float2 *xt = h->xt32.as<float2>();
     hipLaunchKernelGGL(k_pack_xf, dim3((unsigned)std::min<int64_t>(2048, ((int64_t)N * RP + 255) / 256)), dim3(256), 0,
-                       s, x, N, R, RP, xt);
+                       s, x, N, R, RP, xt, cols, j0, gate);
     auto timed = [&](int kind, double bytes, double flops, auto &&launch) {
         if (!pf.on) {
             launch();
             return;
         }
-        Profiler::Entry e{pf.get(), pf.get(), kind, bytes, flops};
+        Profiler::Entry e{pf.get(), pf.get(), kind, bytes, flops, gate ? pf.cur_slot : -1};
         ARH_HIP(hipEventRecord(e.a, s));
         launch();
         ARH_HIP(hipEventRecord(e.b, s));
@@ -561,46 +575,51 @@ static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, hipStrea
     if (nu)
         timed(PROF_LR32, 8.0 * h->v_total + 16.0 * R * (N + h->t_total), 8.0 * R * h->v_total, [&] {
             hipLaunchKernelGGL(k_lr_mfma<NT>, dim3(nb + (nu - nb + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s,
-                               h->d_stacks.as<VStack>(), h->d_p1units.as<int2>(), nb, nu, h->vs32.as<float2>(), xt);
+                               h->d_stacks.as<VStack>(), h->d_p1units.as<int2>(), nb, nu, h->vs32.as<float2>(), xt,
+                               gate);
         });
     const int64_t ent = g->dense_entries + h->u_total;
     float *part = h->nslots ? h->part32.as<float>() : nullptr;
     const int ntask = (int)h->rtasks.size(), nsp = (int)h->rsplit.size();
+    const double *xd = (const double *)x;
+    double *yd = (double *)y;
     timed(PROF_ROWS32, 8.0 * ent + 16.0 * R * (2.0 * N + h->t_total), 8.0 * R * ent, [&] {
         if (maxrt <= 2)
             hipLaunchKernelGGL((k_rows_mfma<NT, 2>), dim3(ntask), dim3(64 * kWaves), 0, s, h->d_rtasks.as<RowTask>(),
                                g->d_leaf_start.as<int>(), h->d_lpanel.as<LeafPanel>(), h->d_psrc.as<int>(),
-                               h->panel32.as<float2>(), (const float2 *)xt, (const double *)x, N, R, (double *)y, part);
+                               h->panel32.as<float2>(), (const float2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         else
             hipLaunchKernelGGL((k_rows_mfma<NT, 4>), dim3(ntask), dim3(64 * kWaves), 0, s, h->d_rtasks.as<RowTask>(),
                                g->d_leaf_start.as<int>(), h->d_lpanel.as<LeafPanel>(), h->d_psrc.as<int>(),
-                               h->panel32.as<float2>(), (const float2 *)xt, (const double *)x, N, R, (double *)y, part);
+                               h->panel32.as<float2>(), (const float2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         if (nsp)
             hipLaunchKernelGGL(k_rows_sum<NT>, dim3(nsp), dim3(256), 0, s, h->d_rsplit.as<int4>(),
-                               g->d_leaf_start.as<int>(), part, (const double *)x, N, R, (double *)y);
+                               g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
     });
     ARH_HIP(hipGetLastError());
 }
 
 int mfma_max_leaf() { return kPartRows; }
 
-// y = (M - I/2) x on the MFMA FP32 layout, right-hand sides in chunks of up to kMaxRC.
-void hmat_apply_mfma(arh_hmat *h, int R, const double2 *x, double2 *y, hipStream_t s) {
+// y = (M - I/2) x on the MFMA FP32 layout, right-hand sides in passes of up to kMaxRC. With a
+// column list (device array of ncols indices into the nrhs columns of x and y), only those columns
+// are computed; the others of y are left untouched. A non-null gate skips every pass when *gate is 0.
+void hmat_apply_mfma(arh_hmat *h, int R, const double2 *x, double2 *y, hipStream_t s, const int *cols, int ncols,
+                     const int *gate) {
     arh_geom *g = h->g;
     ARH_CHECK(g->max_leaf <= mfma_max_leaf(), ARH_EINVAL, "MFMA product supports leaves of at most 64 elements");
     ARH_CHECK(h->panel32.p, ARH_ESTATE, "the H-matrix has no MFMA FP32 layout");
     const int maxrt = (g->max_leaf + 15) / 16;
-    const int64_t N = g->N;
-    for (int j0 = 0; j0 < R; j0 += kMaxRC) {
-        const int Rc = std::min(kMaxRC, R - j0);
-        const double2 *xc = x + j0 * N;
-        double2 *yc = y + j0 * N;
+    const int nc = cols ? ncols : R;
+    ARH_CHECK(nc >= 0 && nc <= R, ARH_EINVAL, "column list longer than the number of right-hand sides");
+    for (int j0 = 0; j0 < nc; j0 += kMaxRC) {
+        const int Rc = std::min(kMaxRC, nc - j0);
         switch ((Rc + 7) / 8) {
-        case 1: launch_nt<1>(h, Rc, xc, yc, s, maxrt); break;
-        case 2: launch_nt<2>(h, Rc, xc, yc, s, maxrt); break;
-        case 3: launch_nt<3>(h, Rc, xc, yc, s, maxrt); break;
-        case 4: launch_nt<4>(h, Rc, xc, yc, s, maxrt); break;
-        default: launch_nt<5>(h, Rc, xc, yc, s, maxrt); break;
+        case 1: launch_nt<1>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 2: launch_nt<2>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 3: launch_nt<3>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 4: launch_nt<4>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        default: launch_nt<5>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
         }
     }
 }

@@ -10,7 +10,10 @@
 //     alpha = rho / (r0^H v); s = r - alpha v; t = A s
 //     w = t^H s / ||t||^2 (0 if ||t|| = 0); x += alpha p + w s; r = s - w t; i++
 // The reductions are two-stage with a fixed split, so every rhs gets the same bits whatever
-// the batch. One small device->host copy per iteration carries ||r||^2 and the next rho.
+// the batch. The loop control (convergence, maxit, breakdown restarts) runs on the device in
+// k_finish, so the host launches K iterations between synchronisations, K chosen from the
+// observed convergence rate; a right-hand side that meets a restart pauses until the host
+// restarts it at the next synchronisation, which leaves its arithmetic unchanged.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,11 +27,18 @@ struct RhsState {
     double2 rho;      // rho of the previous iteration (Eigen's rho_old at the next step)
     double2 rho_new;  // r0^H r for the coming iteration
     double2 alpha, w;
-    double r0sq, rhs_sq, rn, pad;
-    int active, restart, pad0, pad1;
+    double r0sq, rhs_sq, rn, tol2;  // tol2 = tol^2 ||b||^2
+    int active;    // iterating
+    int restart;   // restart pending (set at init, or by k_finish on breakdown: then paused)
+    int it;        // iterations counted toward maxit (reset at the first restart)
+    int iters;     // iterations done
+    int nrestart;  // restarts done
+    int maxit;
+    int pad[2];
 };
 
 constexpr int kVT = 256;      // threads per block of vector kernels
+constexpr int kMaxK = 8;      // iterations per host synchronisation, at most
 constexpr int kMaxNB = 128;   // blocks per rhs of a reduction
 
 __device__ __forceinline__ void block_reduce4(double v[4], double *sh) {
@@ -158,22 +168,51 @@ __global__ __launch_bounds__(kVT) void k_xr_update(const RhsState *__restrict__ 
         for (int k = 0; k < 4; k++) part_rr[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
 }
 
-// end of an iteration: commit alpha, w, rho; rho_new = r0^H r, rn = |r|^2
-__global__ void k_finish(RhsState *__restrict__ S, int R, int NB, const double *__restrict__ part_r0v,
-                         const double *__restrict__ part_ts, const double *__restrict__ part_rr) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= R || !S[j].active) return;
-    RhsState st = S[j];
-    double d[4], e[4], f[4];
-    sum_parts(part_r0v + (int64_t)j * NB * 4, NB, d);
-    sum_parts(part_ts + (int64_t)j * NB * 4, NB, e);
-    sum_parts(part_rr + (int64_t)j * NB * 4, NB, f);
-    st.alpha = cdiv(st.rho_new, make_double2(d[0], d[1]));
-    st.w = e[2] > 0.0 ? make_double2(e[0] / e[2], e[1] / e[2]) : make_double2(0.0, 0.0);
-    st.rho = st.rho_new;
-    st.rho_new = make_double2(f[0], f[1]);
-    st.rn = f[2];
-    S[j] = st;
+// end of an iteration: commit alpha, w, rho; rho_new = r0^H r, rn = |r|^2; then Eigen's loop
+// control for the next iteration: stop when ||r||^2 <= tol2 or it == maxit, pause for a restart
+// when |r0^H r| < eps^2 ||r0||^2. Optional history of ||r|| / ||b||. One workgroup; gate[0]
+// receives the number of right-hand sides still active.
+__global__ __launch_bounds__(256) void k_finish(RhsState *__restrict__ S, int R, int NB,
+                                                const double *__restrict__ part_r0v,
+                                                const double *__restrict__ part_ts, const double *__restrict__ part_rr,
+                                                double *__restrict__ hist, int hist_cap, int *__restrict__ gate,
+                                                int *__restrict__ gate_hist) {
+    __shared__ int cnt[256];
+    const double eps2 = 4.930380657631324e-32;  // (2^-52)^2, Eigen's NumTraits<double>::epsilon() squared
+    int nact = 0;
+    for (int j = threadIdx.x; j < R; j += blockDim.x) {
+        RhsState st = S[j];
+        if (!st.active) continue;
+        double d[4], e[4], f[4];
+        sum_parts(part_r0v + (int64_t)j * NB * 4, NB, d);
+        sum_parts(part_ts + (int64_t)j * NB * 4, NB, e);
+        sum_parts(part_rr + (int64_t)j * NB * 4, NB, f);
+        st.alpha = cdiv(st.rho_new, make_double2(d[0], d[1]));
+        st.w = e[2] > 0.0 ? make_double2(e[0] / e[2], e[1] / e[2]) : make_double2(0.0, 0.0);
+        st.rho = st.rho_new;
+        st.rho_new = make_double2(f[0], f[1]);
+        st.rn = f[2];
+        st.it++;
+        st.iters++;
+        if (hist && st.iters < hist_cap) hist[(int64_t)j * hist_cap + st.iters] = sqrt(st.rn / st.rhs_sq);
+        if (!(st.rn > st.tol2 && st.it < st.maxit)) {
+            st.active = 0;
+        } else if (hypot(st.rho_new.x, st.rho_new.y) < eps2 * st.r0sq) {
+            st.active = 0;
+            st.restart = 1;
+        } else {
+            nact++;
+        }
+        S[j] = st;
+    }
+    cnt[threadIdx.x] = nact;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int k = 0; k < (int)blockDim.x; k++) c += cnt[k];
+        gate[0] = c;
+        gate_hist[0] = c;
+    }
 }
 
 // (re)start of rhs j: r = b - Ax (Ax given, or zero), r0 = r, partials of |r|^2 (and b^2 on init)
@@ -200,7 +239,9 @@ __global__ __launch_bounds__(kVT) void k_restart(const RhsState *__restrict__ S,
         for (int k = 0; k < 4; k++) part[((int64_t)j * NB + blockIdx.x) * 4 + k] = v[k];
 }
 
-// full restart: rho = r0sq = |r|^2; fresh: also alpha = w = rho_old = 1, rhs_sq, and the caller zeroes p, v
+// full restart: rho = r0sq = |r|^2; fresh: also alpha = w = rho_old = 1, rhs_sq, and the caller
+// zeroes p, v (the host decides activity after a fresh start); a breakdown restart resumes the
+// iteration (active again) and resets the iteration count on the first restart (Eigen)
 __global__ void k_restart_finish(RhsState *__restrict__ S, int R, int NB, const double *__restrict__ part,
                                  int fresh) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,6 +257,9 @@ __global__ void k_restart_finish(RhsState *__restrict__ S, int R, int NB, const 
         st.rho = make_double2(1.0, 0.0);
         st.alpha = make_double2(1.0, 0.0);
         st.w = make_double2(1.0, 0.0);
+    } else {
+        if (st.nrestart++ == 0) st.it = 0;
+        st.active = 1;
     }
     st.restart = 0;
     S[j] = st;
@@ -264,16 +308,18 @@ __global__ __launch_bounds__(kVT) void k_resid(int N, int NB, const double2 *__r
 namespace {
 
 struct Work {
-    DBuf buf;
+    DBuf buf, hbuf;
     double2 *r, *r0, *p, *v, *s, *t, *Ax;
     double *part;
     RhsState *S;
     int *mask;
+    int *ctl;  // [0]: gate (active count), [1..]: column list
+    int *ghist;  // active count after each iteration of a batch
     void init(int64_t N, int R, int NB) {
         const int64_t vec = N * R;
         const size_t vb = sizeof(double2) * vec;
         const size_t pb = sizeof(double) * 4 * (size_t)R * NB;
-        size_t total = 7 * vb + 4 * pb + sizeof(RhsState) * R + sizeof(int) * R + 7 * 256;
+        size_t total = 7 * vb + 4 * pb + sizeof(RhsState) * R + sizeof(int) * (2 * R + 1 + kMaxK) + 9 * 256;
         buf.alloc(total);
         char *c = buf.as<char>();
         auto take = [&](size_t n) {
@@ -286,6 +332,8 @@ struct Work {
         part = (double *)take(4 * pb);
         S = (RhsState *)take(sizeof(RhsState) * R);
         mask = (int *)take(sizeof(int) * R);
+        ctl = (int *)take(sizeof(int) * (R + 1));
+        ghist = (int *)take(sizeof(int) * kMaxK);
     }
 };
 
@@ -304,16 +352,24 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
     const int N = h->g->N;
     const dim3 gv(NB, R), bv(kVT);
     const int pblk = (R + 63) / 64;
+    const int zblk = std::max(1, std::min(1024, (N + 255) / 256));
     double *P1 = W.part, *P2 = W.part + 4 * (size_t)R * NB, *P3 = W.part + 8 * (size_t)R * NB;
+    const bool hist_on = record_hist && st && st->hist && st->hist_len && o.hist_cap > 0;
+    const int cap = hist_on ? o.hist_cap : 0;
+    double *hist_dev = nullptr;
+    if (hist_on) {
+        W.hbuf.alloc(sizeof(double) * (size_t)R * cap);
+        hist_dev = W.hbuf.as<double>();
+    }
     std::vector<RhsState> hs(R);
     for (int j = 0; j < R; j++) {
         std::memset(&hs[j], 0, sizeof(RhsState));
         hs[j].restart = want[j] ? 1 : 0;
     }
     ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, x, 1);
-    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, W.p, 1);
-    hipLaunchKernelGGL(k_zero_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s, W.S, N, W.v, 1);
+    hipLaunchKernelGGL(k_zero_cols, dim3(zblk, R), dim3(256), 0, s, W.S, N, x, 1);
+    hipLaunchKernelGGL(k_zero_cols, dim3(zblk, R), dim3(256), 0, s, W.S, N, W.p, 1);
+    hipLaunchKernelGGL(k_zero_cols, dim3(zblk, R), dim3(256), 0, s, W.S, N, W.v, 1);
     hipLaunchKernelGGL(k_restart, gv, bv, 0, s, W.S, N, NB, b, (const double2 *)nullptr, W.r, W.r0, P1);
     hipLaunchKernelGGL(k_restart_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, 1);
     ARH_HIP(hipGetLastError());
@@ -321,8 +377,7 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
     ARH_HIP(hipStreamSynchronize(s));
     iters.assign(R, 0);
     err.assign(R, 0.0);
-    std::vector<int> restarts(R, 0);
-    std::vector<double> tol2(R);
+    std::vector<double> hist0(R, 0.0);
     int nact = 0;
     for (int j = 0; j < R; j++) {
         hs[j].active = 0;
@@ -331,65 +386,104 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
             err[j] = 0.0;
             continue;
         }
-        tol2[j] = tolj[j] * tolj[j] * hs[j].rhs_sq;
-        if (record_hist && st && st->hist && st->hist_len && o.hist_cap > 0) {
-            int &hl = st->hist_len[j];
-            if (hl < o.hist_cap) st->hist[(size_t)j * o.hist_cap + hl++] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
-        }
-        if (hs[j].rn > tol2[j] && maxit > 0) hs[j].active = 1, nact++;
+        hs[j].tol2 = tolj[j] * tolj[j] * hs[j].rhs_sq;
+        hs[j].maxit = maxit;
+        hist0[j] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
+        if (hs[j].rn > hs[j].tol2 && maxit > 0) hs[j].active = 1, nact++;
         err[j] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
     }
-    const double eps2 = std::numeric_limits<double>::epsilon() * std::numeric_limits<double>::epsilon();
-    std::vector<int> it(R, 0);
+    ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
+    // K from the observed convergence: iterations until the fastest active rhs is predicted to
+    // reach its tolerance (so that few iterations run past convergence), 1..kMaxK
+    std::vector<double> e_prev(R, 0.0);
+    std::vector<int> i_prev(R, 0);
+    for (int j = 0; j < R; j++) e_prev[j] = err[j];
+    int K = 2;
+    // separate host buffers for the two lists: a list is rewritten only after the stream has been
+    // synchronised since its last copy
+    std::vector<int> ctl(R + 1), ctl_rs(R + 1);
     while (nact > 0) {
-        // restart check (top of Eigen's loop)
-        bool any_restart = false;
-        for (int j = 0; j < R; j++) {
-            hs[j].restart = 0;
-            if (hs[j].active && std::hypot(hs[j].rho_new.x, hs[j].rho_new.y) < eps2 * hs[j].r0sq) {
-                hs[j].restart = 1;
-                any_restart = true;
-            }
+        int nc = 0;
+        for (int j = 0; j < R; j++)
+            if (hs[j].active) ctl[1 + nc++] = j;
+        ctl[0] = nc;
+        ARH_HIP(hipMemcpyAsync(W.ctl, ctl.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice, s));
+        const int *cols = W.ctl + 1, *gate = W.ctl;
+        for (int k = 0; k < K; k++) {
+            ctx->prof.cur_slot = k;
+            hipLaunchKernelGGL(k_p_update, gv, bv, 0, s, W.S, N, NB, W.r, W.p, W.v);
+            hmat_apply(h, transpose, prec, R, W.p, W.v, s, cols, nc, gate);
+            hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.r0, W.v, (const double2 *)nullptr, P1);
+            hipLaunchKernelGGL(k_s_update, gv, bv, 0, s, W.S, N, NB, P1, W.r, W.v, W.s);
+            hmat_apply(h, transpose, prec, R, W.s, W.t, s, cols, nc, gate);
+            hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.t, W.s, W.t, P2);
+            hipLaunchKernelGGL(k_xr_update, gv, bv, 0, s, W.S, N, NB, P1, P2, W.p, W.s, W.t, W.r0, x, W.r, P3);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, W.S, R, NB, P1, P2, P3, hist_dev, cap, W.ctl,
+                               W.ghist + k);
+            ARH_HIP(hipGetLastError());
         }
-        ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
-        if (any_restart) {
-            hmat_apply(h, transpose, prec, R, x, W.Ax, s);
+        ctx->prof.cur_slot = -1;
+        int gh[kMaxK];
+        ARH_HIP(hipMemcpyAsync(gh, W.ghist, sizeof(int) * K, hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipMemcpyAsync(hs.data(), W.S, sizeof(RhsState) * R, hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipStreamSynchronize(s));
+        if (ctx->prof.on) {
+            // iteration k's products ran iff some rhs was active after iteration k - 1
+            ctx->prof.slot_ok.assign(K, 1);
+            for (int k = 1; k < K; k++) ctx->prof.slot_ok[k] = gh[k - 1] > 0;
+            ctx->prof.flush(false);
+            ctx->prof.slot_ok.clear();
+        }
+        // breakdown restarts: r = b - A x, r0 = r for the paused right-hand sides
+        int nrs = 0;
+        for (int j = 0; j < R; j++)
+            if (hs[j].restart) ctl_rs[1 + nrs++] = j;
+        if (nrs) {
+            ctl_rs[0] = nrs;
+            ARH_HIP(hipMemcpyAsync(W.ctl, ctl_rs.data(), sizeof(int) * (nrs + 1), hipMemcpyHostToDevice, s));
+            hmat_apply(h, transpose, prec, R, x, W.Ax, s, W.ctl + 1, nrs, W.ctl);
             if (st) st->matvecs++;
             hipLaunchKernelGGL(k_restart, gv, bv, 0, s, W.S, N, NB, b, W.Ax, W.r, W.r0, P1);
             hipLaunchKernelGGL(k_restart_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, 0);
+            ARH_HIP(hipGetLastError());
             for (int j = 0; j < R; j++)
                 if (hs[j].restart) {
-                    if (restarts[j]++ == 0) it[j] = 0;
+                    hs[j].restart = 0;
+                    hs[j].active = 1;
                     if (st) st->restarts++;
                 }
         }
-        hipLaunchKernelGGL(k_p_update, gv, bv, 0, s, W.S, N, NB, W.r, W.p, W.v);
-        hmat_apply(h, transpose, prec, R, W.p, W.v, s);
-        hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.r0, W.v, (const double2 *)nullptr, P1);
-        hipLaunchKernelGGL(k_s_update, gv, bv, 0, s, W.S, N, NB, P1, W.r, W.v, W.s);
-        hmat_apply(h, transpose, prec, R, W.s, W.t, s);
-        hipLaunchKernelGGL(k_dot, gv, bv, 0, s, W.S, N, NB, W.t, W.s, W.t, P2);
-        hipLaunchKernelGGL(k_xr_update, gv, bv, 0, s, W.S, N, NB, P1, P2, W.p, W.s, W.t, W.r0, x, W.r, P3);
-        hipLaunchKernelGGL(k_finish, dim3(pblk), dim3(64), 0, s, W.S, R, NB, P1, P2, P3);
-        ARH_HIP(hipGetLastError());
-        if (st) st->matvecs += 2;
-        ARH_HIP(hipMemcpyAsync(hs.data(), W.S, sizeof(RhsState) * R, hipMemcpyDeviceToHost, s));
-        ARH_HIP(hipStreamSynchronize(s));
-        if (ctx->prof.on) ctx->prof.flush(false);
+        if (st) st->matvecs += 2 * K;
         nact = 0;
+        double kpred = kMaxK;
         for (int j = 0; j < R; j++) {
-            if (!hs[j].active) continue;
-            it[j]++;
-            iters[j]++;
+            if (!want[j] || hs[j].rhs_sq == 0.0) continue;
             err[j] = std::sqrt(hs[j].rn / hs[j].rhs_sq);
-            if (record_hist && st && st->hist && st->hist_len && o.hist_cap > 0) {
-                int &hl = st->hist_len[j];
-                if (hl < o.hist_cap) st->hist[(size_t)j * o.hist_cap + hl++] = err[j];
+            if (!hs[j].active) continue;
+            nact++;
+            // geometric extrapolation of the residual since the previous synchronisation
+            const int di = hs[j].iters - i_prev[j];
+            if (di > 0 && err[j] > 0.0 && e_prev[j] > err[j]) {
+                const double rate = std::log(e_prev[j] / err[j]) / di;
+                kpred = std::min(kpred, std::log(err[j] / tolj[j]) / rate);
             }
-            if (hs[j].rn > tol2[j] && it[j] < maxit)
-                nact++;
-            else
-                hs[j].active = 0;
+            e_prev[j] = err[j];
+            i_prev[j] = hs[j].iters;
+        }
+        K = std::max(1, std::min(kMaxK, (int)std::floor(kpred)));
+    }
+    for (int j = 0; j < R; j++) iters[j] = hs[j].iters;
+    if (hist_on) {
+        std::vector<double> hh((size_t)R * cap);
+        ARH_HIP(hipMemcpyAsync(hh.data(), hist_dev, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < R; j++) {
+            if (!want[j] || hs[j].rhs_sq == 0.0) continue;
+            const int n = std::min(cap, 1 + hs[j].iters);
+            double *dst = st->hist + (size_t)j * cap;
+            dst[0] = hist0[j];
+            for (int k = 1; k < n; k++) dst[k] = hh[(size_t)j * cap + k];
+            st->hist_len[j] = n;
         }
     }
 }
