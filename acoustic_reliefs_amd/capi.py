@@ -115,6 +115,8 @@ SIGNATURES = [
                                   ctypes.POINTER(ctypes.c_int64)]),
     ("arh_server_wait", c_int, [c_void_p, ctypes.c_int64, P_D, ctypes.POINTER(RequestInfo)]),
     ("arh_server_poll", c_int, [c_void_p, ctypes.c_int64, ctypes.POINTER(RequestInfo)]),
+    ("arh_server_profile", c_int, [c_void_p, c_int]),
+    ("arh_server_get_profile", c_int, [c_void_p, ctypes.POINTER(Profile)]),
     ("arh_server_destroy", None, [c_void_p]),
 ]
 
@@ -474,6 +476,14 @@ class Server:
         info = RequestInfo()
         _check(lib().arh_server_poll(self.h, ctypes.c_int64(rid), ctypes.byref(info)))
         return info.as_dict()
+
+    def profile(self, enable=True):
+        _check(lib().arh_server_profile(self.h, int(enable)))
+
+    def get_profile(self):
+        p = Profile()
+        _check(lib().arh_server_get_profile(self.h, ctypes.byref(p)))
+        return p.as_dict()
 
     def close(self):
         if getattr(self, "h", None) and not _EXITING:

@@ -58,6 +58,8 @@ struct arh_server_s {
     int64_t next_id = 1;
     int live = 0;  // workers that started
     bool stop = false;
+    bool profiling = false;
+    arh_profile prof{};
     Clock::time_point t0 = Clock::now();
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -104,6 +106,7 @@ static void worker(arh_server *s, int wid) {
     }
     int cached = -1;
     arh_geom *geom = nullptr;
+    bool prof_on = false;
     for (;;) {
         Request *r = nullptr;
         {
@@ -120,7 +123,9 @@ static void worker(arh_server *s, int wid) {
             s->used += r->footprint;
             r->info.status = 1;
             r->info.t_start = s->now();
+            prof_on = s->profiling;
         }
+        arh_ctx_profile(ctx, prof_on ? 1 : 0);
         int rc = ARH_OK;
         const SMesh &m = *s->meshes[r->mesh];
         if (cached != r->mesh) {
@@ -139,8 +144,19 @@ static void worker(arh_server *s, int wid) {
             rc = arh_band_solve(geom, r->k, r->aca_tol, 100, r->storage, r->kind, r->nrhs, r->vecs.data(),
                                 r->L.data(), (int)r->L.size() / 3, &o, d.data(), nullptr, nullptr, &st, nullptr);
         }
+        arh_profile pr{};
+        if (prof_on) arh_ctx_get_profile(ctx, &pr);
         {
             std::lock_guard<std::mutex> lk(s->mu);
+            if (prof_on && s->profiling) {
+                auto add = [](arh_kernel_prof &a, const arh_kernel_prof &b) {
+                    a.launches += b.launches, a.ms += b.ms, a.bytes += b.bytes, a.flops += b.flops;
+                };
+                add(s->prof.rows_fp64, pr.rows_fp64);
+                add(s->prof.lowrank_fp64, pr.lowrank_fp64);
+                add(s->prof.rows_fp32, pr.rows_fp32);
+                add(s->prof.lowrank_fp32, pr.lowrank_fp32);
+            }
             s->used -= r->footprint;
             r->d = std::move(d);
             r->info.iters = st.iters;
@@ -265,6 +281,23 @@ int arh_server_poll(arh_server *s, int64_t id, arh_request_info *info) {
         auto it = s->reqs.find(id);
         ARH_CHECK(it != s->reqs.end(), ARH_EINVAL, "unknown request id");
         *info = it->second->info;
+    });
+}
+
+int arh_server_profile(arh_server *s, int enable) {
+    return sguard([&] {
+        ARH_CHECK(s, ARH_EINVAL, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->profiling = enable != 0;
+        if (enable) s->prof = arh_profile{};
+    });
+}
+
+int arh_server_get_profile(arh_server *s, arh_profile *out) {
+    return sguard([&] {
+        ARH_CHECK(s && out, ARH_EINVAL, "NULL argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->prof;
     });
 }
 

@@ -241,6 +241,10 @@ int arh_server_submit(arh_server *s, int mesh_id, double k, double aca_tol, int 
 /* Block until the request is done; d (host, nrhs) receives the diffusion coefficients. */
 int arh_server_wait(arh_server *s, int64_t request_id, double *d, arh_request_info *info);
 int arh_server_poll(arh_server *s, int64_t request_id, arh_request_info *info);
+/* Kernel profile of the server's product kernels (as arh_ctx_profile): enable clears the totals;
+ * each request admitted while enabled adds its launches when it completes. */
+int arh_server_profile(arh_server *s, int enable);
+int arh_server_get_profile(arh_server *s, arh_profile *out);
 void arh_server_destroy(arh_server *s);
 
 #ifdef __cplusplus
