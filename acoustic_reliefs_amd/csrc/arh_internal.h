@@ -173,7 +173,14 @@ struct VStack {
     int64_t t_row;  // row of the stack's first intermediate in the packed [x; t] buffer
 };
 
-// MFMA product: one row leaf's fused FP32 panel (dense columns, then the U columns of the low-rank
+// MFMA product layout builder: block blk's V goes to stacked rows starting at dst (c = 0, p = 0)
+// of a stack with rtot rows.
+struct VMap {
+    int blk, rtot;
+    int64_t dst;
+};
+
+// MFMA product: one row leaf's fused panel (dense columns, then the U columns of the low-rank
 // blocks covering it, then zero columns up to a multiple of 4), m x ncol column-major at `off`;
 // the packed [x; t] row feeding panel column c is src[col0 + c].
 struct LeafPanel {
@@ -253,20 +260,24 @@ struct arh_hmat_s {
     std::vector<int> active_lr;  // lr blocks with rank > 0
     arh::DBuf d_active_lr;
     arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
-    // MFMA multi-rhs product (FP32 copy in its own layout): fused row-leaf panels, V stacks, the
-    // packed rhs + intermediates buffer ([x; t] rows of RP complex), phase-1 units (stack, 32-row
-    // tile; stacks with many columns first, one workgroup each), row tasks (largest first), split
-    // leaves (leaf, first partial slot, task count) and their partial tiles
-    arh::DBuf panel32, vs32, xt32, part32;
-    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit;
+    // MFMA multi-rhs product: operator copies in their own layout, [0] FP32 (built at assembly
+    // for mixed storage), [1] FP64 (built on first multi-rhs FP64 product): fused row-leaf panels,
+    // V stacks, the packed rhs + intermediates buffer ([x; t] rows of RP complex) and the split
+    // leaves' partial tiles. Shared work lists: stacks, phase-1 units (stack, 32-row tile; stacks
+    // with many columns first, one workgroup each), per-leaf panels and their source rows, row
+    // tasks, split leaves (leaf, first partial slot, task count), layout builder maps.
+    arh::DBuf mf_panel[2], mf_vs[2], mf_xt[2], mf_part[2];
+    bool mf_built[2] = {false, false};
+    bool mf_lists = false;
+    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit, d_vmaps, d_csrc;
     std::vector<VStack> stacks;
     std::vector<int2> p1units;
-    int p1big = 0;
+    int p1big = 0, nvmaps = 0;
     std::vector<LeafPanel> lpanel;
     std::vector<RowTask> rtasks;
     std::vector<int4> rsplit;
     int nslots = 0;
-    int64_t panel32_total = 0, vs32_total = 0;
+    int64_t panel_total = 0, vs_total = 0;
     bool generic32 = false;  // dense32 / U32 / V32 (the non-MFMA FP32 product) built
     // FP64 product: phase-1 units (block, 4-row group), phase-2 units per leaf, per-unit partials
     std::vector<MvUnit> mv_units;
@@ -291,9 +302,9 @@ void make_generic_fp32(arh_hmat *h);
 void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s,
                 const int *cols = nullptr, int ncols = 0, const int *gate = nullptr);
 // matvec_mfma.hip
-void hmat_apply_mfma(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s, const int *cols = nullptr,
-                     int ncols = 0, const int *gate = nullptr);
-void mfma_prepare(arh_hmat *h);
+void hmat_apply_mfma(arh_hmat *h, int fp64, int nrhs, const double2 *x, double2 *y, hipStream_t s,
+                     const int *cols = nullptr, int ncols = 0, const int *gate = nullptr);
+void mfma_prepare(arh_hmat *h, int fp64);
 // matvec64.hip
 void hmat_apply64(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
 int mfma_max_leaf();
