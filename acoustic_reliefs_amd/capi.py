@@ -34,7 +34,7 @@ class SolveOpts(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", c_int), ("matvecs", c_int), ("refinements", c_int), ("restarts", c_int),
                 ("max_err", c_double), ("max_true_res", c_double), ("iters_per_rhs", P_I), ("true_res", P_D),
-                ("hist", P_D), ("hist_len", P_I)]
+                ("hist", P_D), ("hist_len", P_I), ("solve_ms", c_double), ("post_ms", c_double)]
 
 
 class HMatInfo(ctypes.Structure):
@@ -68,7 +68,8 @@ class ServerOpts(ctypes.Structure):
 
 class RequestInfo(ctypes.Structure):
     _fields_ = [("status", c_int), ("nrhs", c_int), ("t_submit", c_double), ("t_start", c_double),
-                ("t_done", c_double), ("footprint", ctypes.c_int64), ("iters", c_int), ("max_true_res", c_double)]
+                ("t_done", c_double), ("footprint", ctypes.c_int64), ("iters", c_int), ("max_true_res", c_double),
+                ("assemble_ms", c_double), ("solve_ms", c_double), ("post_ms", c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -354,7 +355,8 @@ class Geometry:
                                     _dp(vecs), _dp(L), LL, ctypes.byref(opts), _dp(d), _ptr(y) if y else None,
                                     _ptr(x) if x else None, ctypes.byref(st), ctypes.byref(info)), self.ctx.h)
         stats = {"iters": st.iters, "matvecs": st.matvecs, "refinements": st.refinements, "restarts": st.restarts,
-                 "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr}
+                 "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr,
+                 "solve_ms": st.solve_ms, "post_ms": st.post_ms}
         return d, x, y, stats, info.as_dict()
 
 
@@ -416,7 +418,8 @@ class HMatrix:
         _check(lib().arh_solve(self.h, int(transpose), int(nrhs), _ptr(b), _ptr(x), ctypes.byref(opts),
                                ctypes.byref(st)), self.geom.ctx.h)
         out = {"iters": st.iters, "matvecs": st.matvecs, "refinements": st.refinements, "restarts": st.restarts,
-               "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr}
+               "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr,
+               "solve_ms": st.solve_ms}
         if hist_cap:
             out["hist"] = [hist[j * hist_cap:j * hist_cap + hl[j]].copy() for j in range(nrhs)]
         return out

@@ -465,10 +465,13 @@ int arh_band_solve(arh_geom *g, double k, double aca_tol, int max_rank, int stor
         // solve (M - I/2) x = -b (adjoint.cpp:95)
         launch_scale(s, bb.as<double2>(), (int64_t)N * nrhs, -1.0);
         solve(h, 0, nrhs, bb.as<double2>(), xo, *opts, stats);
+        const auto t1 = std::chrono::steady_clock::now();
         launch_receivers(g, k, Ld.as<double>(), LL, 0, nrhs, xo, yo);
         launch_diffusion(s, LL, nrhs, yo, dd.as<double>(), nullptr, nullptr);
         ARH_HIP(hipMemcpyAsync(d, dd.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
         ARH_HIP(hipStreamSynchronize(s));
+        if (stats)
+            stats->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
         if (info) *info = h->info;
     });
     arh_hmat_destroy(h);

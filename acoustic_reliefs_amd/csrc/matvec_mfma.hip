@@ -660,10 +660,10 @@ static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, const in
         } else {
             ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 32 elements");
         }
-        if (nsp)
-            hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp), dim3(256), 0, s, h->d_rsplit.as<int4>(),
-                               g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
     });
+    if (nsp)
+        hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp), dim3(256), 0, s, h->d_rsplit.as<int4>(),
+                           g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
     ARH_HIP(hipGetLastError());
 }
 

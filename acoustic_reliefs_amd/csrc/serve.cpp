@@ -136,13 +136,14 @@ static void worker(arh_server *s, int wid) {
             if (rc == ARH_OK) cached = r->mesh;
         }
         arh_solve_stats st{};
+        arh_hmat_info hi{};
         std::vector<double> d(r->nrhs, 0.0);
         if (rc == ARH_OK) {
             arh_solve_opts o{};
             o.tol = r->tol;
             o.true_residual = 1;
             rc = arh_band_solve(geom, r->k, r->aca_tol, 100, r->storage, r->kind, r->nrhs, r->vecs.data(),
-                                r->L.data(), (int)r->L.size() / 3, &o, d.data(), nullptr, nullptr, &st, nullptr);
+                                r->L.data(), (int)r->L.size() / 3, &o, d.data(), nullptr, nullptr, &st, &hi);
         }
         arh_profile pr{};
         if (prof_on) arh_ctx_get_profile(ctx, &pr);
@@ -161,6 +162,9 @@ static void worker(arh_server *s, int wid) {
             r->d = std::move(d);
             r->info.iters = st.iters;
             r->info.max_true_res = st.max_true_res;
+            r->info.assemble_ms = hi.assemble_ms;
+            r->info.solve_ms = st.solve_ms;
+            r->info.post_ms = st.post_ms;
             r->info.t_done = s->now();
             if (rc != ARH_OK) r->err = arh_last_error(ctx);
             r->info.status = rc == ARH_OK ? 2 : rc;

@@ -15,6 +15,7 @@
 // observed convergence rate; a right-hand side that meets a restart pauses until the host
 // restarts it at the next synchronisation, which leaves its arithmetic unchanged.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -518,6 +519,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
     const double tol = o.tol;
     ARH_CHECK(R >= 1, ARH_EINVAL, "nrhs must be >= 1");
     ARH_CHECK(tol >= 0.0, ARH_EINVAL, "tol must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
     Work W;
     W.init(N, R, NB);
     if (st) {
@@ -604,6 +606,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             if (st->iters_per_rhs) st->iters_per_rhs[j] = iters_tot[j];
             if (st->true_res) st->true_res[j] = tres[j];
         }
+        st->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (o.fail_on_noconv) {
         // true-residual modes are judged on the FP64 true residual, Eigen semantics on the recursive one

@@ -105,11 +105,24 @@ def run_fp64_plain(g, bands, dirs, L, capi, ctx, lat):
     return out
 
 
-def cpu_baseline(P, E, dirs, L, budget_s=20.0):
-    """CPU oracle (FP64, all host threads) on one band (1000 Hz): assembly + listener matrix + as
-    many single-rhs solves as fit the budget; the assembly is amortised over 37 solves as in the
-    full workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, E, dirs, L, threads=None, budget_s=20.0):
+    """CPU oracle (FP64, OpenMP) on one band (1000 Hz): assembly + listener matrix + as many
+    single-rhs solves as fit the budget; the assembly is amortised over 37 solves as in the full
+    workload. threads=None: all the threads OpenMP is given."""
     from oracle import oracle as O
+    if threads:
+        O.set_threads(threads)
     threads = O.num_threads()
     Er, dirb, appb, _ = O.setup(P, E, 32, 1.5)
     oc = O.Ctx(P, Er, dirb, appb)
@@ -128,9 +141,58 @@ def cpu_baseline(P, E, dirs, L, budget_s=20.0):
         n += 1
     per_solve = t_asm / len(dirs) + t_solve / n
     return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "oracle (CPU restatement, FP64, OpenMP %d threads): 1000 Hz band of the same relief, "
                       "assembly %.2f s (amortised over 37 solves) + %d single-rhs solves %.2f s" % (
                           threads, t_asm, n, t_solve)}
+
+
+def isolated_roofline(g, capi, ctx, f=4000.0, reps=20):
+    """The dominant kernel on its own stream, nothing else in flight: the 37-rhs FP32 MFMA product of
+    the f Hz band's operator (and the FP64 one), HIP-event timed."""
+    k = 2 * np.pi * f / C0
+    h = g.assemble(k, 1e-4, 100, capi.ARH_MIXED)
+    R = 37
+    rng = np.random.default_rng(0)
+    X = ctx.to_device(rng.standard_normal((R, g.N)) + 1j * rng.standard_normal((R, g.N)))
+    Y = ctx.empty((R, g.N))
+    out = {}
+    N, T = g.N, h.info()["sum_rank"]
+    for name, prec, kern, peak in (("fp32", capi.ARH_MIXED, "rows_fp32", FP32_PEAK_TFLOPS),
+                                   ("fp64", capi.ARH_FP64, "rows_fp64", FP64_PEAK_TFLOPS)):
+        h.matvec(X, Y, nrhs=R, precision=prec)
+        ctx.synchronize()
+        ctx.profile(True)
+        for _ in range(reps):
+            h.matvec(X, Y, nrhs=R, precision=prec)
+        p = ctx.get_profile()
+        ctx.profile(False)
+        rw = p[kern]
+        lr = p[kern.replace("rows", "lowrank")]
+        ach = rw["flops"] / (rw["ms"] * 1e-3) / 1e12
+        out[name] = {"kernel": "k_rows_mfma<%s, 5, 2> (%g Hz band, %d rhs, N=%d, %d intermediates rows)" % (
+                         "float" if name == "fp32" else "double", f, R, N, T),
+                     "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                     "traffic": None, "avg_launch_us": 1000 * rw["ms"] / rw["launches"], "launches": rw["launches"],
+                     "flops_per_launch": rw["flops"] / rw["launches"], "bytes_per_launch": rw["bytes"] / rw["launches"],
+                     "lowrank_phase_us": 1000 * lr["ms"] / max(1, lr["launches"]),
+                     "hbm_GBs": rw["bytes"] / (rw["ms"] * 1e-3) / 1e9,
+                     "timed": "HIP events around each launch on its stream, %d launches, nothing else in flight"
+                              % rw["launches"]}
+    del h
+    return out
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of nbytes (torch), for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
 def main():
@@ -142,16 +204,31 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--freqs", type=str, default="", help="comma-separated band frequencies (Hz) instead of --bands")
     ap.add_argument("--workers", type=int, default=4, help="server workers (concurrent band solves)")
     ap.add_argument("--sequential", action="store_true", help="solve the bands one after another (no server)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the isolated timing of the dominant kernel (the roofline figures)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.roofline_only:
+        from acoustic_reliefs_amd import capi
+        P, E, dirs, L, _ = make_workload(args.grid)
+        ctx = capi.Context(0)
+        g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
+        print(json.dumps(isolated_roofline(g, capi, ctx)), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     from acoustic_reliefs_amd import capi
     P, E, dirs, L, bands = make_workload(args.grid)
-    bands = bands[:args.bands]
+    bands = [float(f) for f in args.freqs.split(",")] if args.freqs else bands[:args.bands]
     ctx = capi.Context(0)
     g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
     nsolve = len(bands) * len(dirs)
@@ -210,19 +287,34 @@ def main():
                           "note": ("a solve's latency is its band's wall time (37 rhs per batched solve)"
                                    if args.sequential else
                                    "a solve's latency is its band request's completion - submission time, all "
-                                   "13 band requests (37 rhs each) submitted together to a %d-worker server"
-                                   % args.workers)},
+                                   "%d band requests (37 rhs each) submitted together to a %d-worker server"
+                                   % (len(bands), args.workers))},
            "max_true_residual": worst_res}
+
+    if not args.sequential:
+        tot = {"assembly": 0.0, "solve": 0.0, "receivers": 0.0}
+        for r in res:
+            tot["assembly"] += r[1]["assemble_ms"]
+            tot["solve"] += r[1]["solve_ms"]
+            tot["receivers"] += r[1]["post_ms"]
+        out["split_ms"] = dict(tot, note="service time of the last step summed over its band requests (they "
+                                         "overlap on the device): H-matrix assembly incl. the FP32 copy, batched "
+                                         "solve, receivers + diffusion coefficients")
 
     rw = prof["rows_fp32"]
     if rw["launches"]:
         ach = rw["flops"] / (rw["ms"] * 1e-3) / 1e12
-        out["roofline"] = {"kernel": "k_rows<float2> (per-leaf dense panel + U segments, 37 rhs)", "bound": "mfma",
-                           "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                           "avg_launch_us": 1000 * rw["ms"] / rw["launches"], "launches": rw["launches"],
-                           "hbm_GBs": rw["bytes"] / (rw["ms"] * 1e-3) / 1e9}
+        out["roofline_served"] = {"kernel": "k_rows_mfma<float> (all pass widths, active rhs of each batched solve)",
+                                  "bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach / FP32_PEAK_TFLOPS,
+                                  "avg_launch_us": 1000 * rw["ms"] / rw["launches"], "launches": rw["launches"],
+                                  "note": "HIP events over the served timed region: the band requests run "
+                                          "concurrently, so a launch shares the GPU and its event interval also "
+                                          "holds the wait for a free slot"}
     out["kernel_profile"] = prof
+    iso = isolated_roofline(g, capi, ctx)
+    out["roofline"] = iso["fp32"]
+    out["roofline_fp64_37rhs"] = iso["fp64"]
 
     if not args.no_fp64:
         ctx.profile(True)
@@ -244,7 +336,7 @@ def main():
         r64 = prof64["rows_fp64"]
         if r64["launches"]:
             gbs = r64["bytes"] / (r64["ms"] * 1e-3) / 1e9
-            out["roofline_fp64_matvec"] = {"kernel": "k_rows<double2> (1 rhs)", "bound": "hbm", "achieved": gbs,
+            out["roofline_fp64_matvec"] = {"kernel": "k_rows_mfma<double> (1 rhs)", "bound": "hbm", "achieved": gbs,
                                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                                            "avg_launch_us": 1000 * r64["ms"] / r64["launches"],
                                            "launches": r64["launches"]}
@@ -253,6 +345,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, E, dirs, L)
         log("cpu baseline:", out["cpu_baseline"])
+        out["cpu_baseline_1core"] = cpu_baseline(P, E, dirs, L, threads=1)
+        log("cpu baseline, 1 thread:", out["cpu_baseline_1core"])
     print(json.dumps(out), flush=True)
 
 

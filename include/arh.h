@@ -72,6 +72,8 @@ typedef struct {
     double *true_res;   /* optional host array [nrhs] */
     double *hist;       /* optional host array [nrhs * hist_cap]: ||r_i|| / ||b|| per iteration */
     int *hist_len;      /* optional host array [nrhs] */
+    double solve_ms;    /* wall time of the solve (out) */
+    double post_ms;     /* arh_band_solve: wall time of receivers + diffusion coefficients (out) */
 } arh_solve_stats;
 
 typedef struct {
@@ -229,6 +231,7 @@ typedef struct {
     int64_t footprint;     /* admitted HBM bytes (estimate) */
     int iters;             /* BiCGSTAB iterations (max over rhs) */
     double max_true_res;   /* FP64 true residual (max over rhs) */
+    double assemble_ms, solve_ms, post_ms; /* split of the service time (wall, on the worker) */
 } arh_request_info;
 
 int arh_server_create(const arh_server_opts *opts, arh_server **out);
