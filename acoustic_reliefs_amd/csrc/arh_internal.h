@@ -165,24 +165,33 @@ struct DenseTask {
     int leaf, c0, nc, pad;
 };
 
-// MFMA product: the low-rank V factors of all blocks with one column cluster, stacked (column-
-// major: Vs[v_off + c rtot + p], p < rtot = sum of the blocks' ranks).
+// MFMA product: the low-rank factors of all blocks with one column cluster (forward: V) or one
+// row cluster (transposed: U^T), stacked row-wise, column-major: Vs[v_off + c rtot + p], p < rtot =
+// the sum of the blocks' ranks, c < n = the cluster's elements starting at cs.
 struct VStack {
     int cs, n, rtot, pad;
-    int64_t v_off;  // offset in the stacked FP32 V
+    int64_t v_off;  // offset in the stacked factor
     int64_t t_row;  // row of the stack's first intermediate in the packed [x; t] buffer
 };
 
-// MFMA product layout builder: block blk's V goes to stacked rows starting at dst (c = 0, p = 0)
-// of a stack with rtot rows.
-struct VMap {
+// MFMA product layout builder: block blk's factor (V forward, U^T transposed) goes to stacked rows
+// starting at dst (c = 0, p = 0) of a stack with rtot rows.
+struct StackMap {
     int blk, rtot;
     int64_t dst;
 };
 
-// MFMA product: one row leaf's fused panel (dense columns, then the U columns of the low-rank
-// blocks covering it, then zero columns up to a multiple of 4), m x ncol column-major at `off`;
-// the packed [x; t] row feeding panel column c is src[col0 + c].
+// MFMA product layout builder: one panel column gathered from the FP64 operator, entries off +
+// i * stride of the dense panels (kind 1), U (kind 2) or V (kind 3); kind 0 is a zero column.
+struct ColSrc {
+    int64_t off;
+    int stride, kind;
+};
+
+// MFMA product: one leaf's fused panel (forward: the row leaf's dense columns, then the U columns
+// of the low-rank blocks covering it; transposed: the column leaf's transposed dense blocks, then
+// the V columns restricted to it), zero columns up to a multiple of 4; m x ncol column-major at
+// `off`; the packed [x; t] row feeding panel column c is src[col0 + c].
 struct LeafPanel {
     int64_t off;
     int col0, ngroups;  // ngroups = ncol / 4
@@ -192,6 +201,27 @@ struct LeafPanel {
 // partial tile when the leaf is split over several tasks, -1 when the task writes y itself.
 struct RowTask {
     int leaf, g0, g1, slot;
+};
+
+// One orientation (forward or transposed) of the MFMA product: operator copies in their own layout
+// ([0] FP32, [1] FP64, each built on first use or at assembly), the packed rhs + intermediates
+// buffer ([x; t] rows of RP complex), split leaves' partial tiles, and the shared work lists:
+// stacks, phase-1 units (stack, 32-row tile; stacks with many columns first, one workgroup each),
+// per-leaf panels and their source rows, row tasks, split leaves (leaf, first partial slot, task
+// count), layout builder maps.
+struct MfmaLayout {
+    arh::DBuf panel[2], vs[2], xt[2], part[2];
+    bool built[2] = {false, false};
+    bool lists = false;
+    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit, d_maps, d_csrc;
+    std::vector<VStack> stacks;
+    std::vector<int2> p1units;
+    int p1big = 0, nmaps = 0;
+    std::vector<LeafPanel> lpanel;
+    std::vector<RowTask> rtasks;
+    std::vector<int4> rsplit;
+    int nslots = 0;
+    int64_t panel_total = 0, vs_total = 0;
 };
 
 // Low-rank block descriptor (device). Offsets are in complex elements.
@@ -221,7 +251,7 @@ struct arh_geom_s {
     int nleaf = 0, max_leaf = 0;
     std::vector<int> leaf_start;  // nleaf + 1
     std::vector<int> leaf_of;     // N
-    arh::DBuf d_leaf_start;
+    arh::DBuf d_leaf_start, d_leaf_of;
     // dense panels: per row leaf, the row's dense blocks side by side (m x ncol column-major)
     std::vector<int> pcol_ptr;        // nleaf + 1: first panel column of each row leaf
     std::vector<int64_t> panel_off;   // nleaf + 1: complex offset of each panel
@@ -260,24 +290,7 @@ struct arh_hmat_s {
     std::vector<int> active_lr;  // lr blocks with rank > 0
     arh::DBuf d_active_lr;
     arh::DBuf tbuf;  // intermediate t / s values (t_total x nrhs c128)
-    // MFMA multi-rhs product: operator copies in their own layout, [0] FP32 (built at assembly
-    // for mixed storage), [1] FP64 (built on first multi-rhs FP64 product): fused row-leaf panels,
-    // V stacks, the packed rhs + intermediates buffer ([x; t] rows of RP complex) and the split
-    // leaves' partial tiles. Shared work lists: stacks, phase-1 units (stack, 32-row tile; stacks
-    // with many columns first, one workgroup each), per-leaf panels and their source rows, row
-    // tasks, split leaves (leaf, first partial slot, task count), layout builder maps.
-    arh::DBuf mf_panel[2], mf_vs[2], mf_xt[2], mf_part[2];
-    bool mf_built[2] = {false, false};
-    bool mf_lists = false;
-    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit, d_vmaps, d_csrc;
-    std::vector<VStack> stacks;
-    std::vector<int2> p1units;
-    int p1big = 0, nvmaps = 0;
-    std::vector<LeafPanel> lpanel;
-    std::vector<RowTask> rtasks;
-    std::vector<int4> rsplit;
-    int nslots = 0;
-    int64_t panel_total = 0, vs_total = 0;
+    MfmaLayout mf[2];  // MFMA product: [0] forward, [1] transposed
     bool generic32 = false;  // dense32 / U32 / V32 (the non-MFMA FP32 product) built
     // FP64 product: phase-1 units (block, 4-row group), phase-2 units per leaf, per-unit partials
     std::vector<MvUnit> mv_units;
@@ -302,9 +315,9 @@ void make_generic_fp32(arh_hmat *h);
 void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s,
                 const int *cols = nullptr, int ncols = 0, const int *gate = nullptr);
 // matvec_mfma.hip
-void hmat_apply_mfma(arh_hmat *h, int fp64, int nrhs, const double2 *x, double2 *y, hipStream_t s,
+void hmat_apply_mfma(arh_hmat *h, int transpose, int fp64, int nrhs, const double2 *x, double2 *y, hipStream_t s,
                      const int *cols = nullptr, int ncols = 0, const int *gate = nullptr);
-void mfma_prepare(arh_hmat *h, int fp64);
+void mfma_prepare(arh_hmat *h, int transpose, int fp64);
 // matvec64.hip
 void hmat_apply64(arh_hmat *h, int nrhs, const double2 *x, double2 *y, hipStream_t s);
 int mfma_max_leaf();

@@ -408,7 +408,7 @@ void make_generic_fp32(arh_hmat *h) {
 // FP32 copy at assembly: the MFMA layout when the leaves fit it, else the generic one.
 void make_fp32_copy(arh_hmat *h) {
     if (h->g->max_leaf <= mfma_max_leaf())
-        mfma_prepare(h, 0);
+        mfma_prepare(h, 0, 0);
     else
         make_generic_fp32(h);
 }

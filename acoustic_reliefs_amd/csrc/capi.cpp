@@ -260,10 +260,12 @@ static void fill_info(arh_hmat *h) {
     I.sum_rank = 0;
     for (auto &B : h->hblocks) I.max_rank = std::max(I.max_rank, B.rank_ref), I.sum_rank += B.rank;
     I.bytes_fp64 = (int64_t)sizeof(double2) * (g->dense_entries + h->u_total + h->v_total);
-    I.bytes_fp32 = (int64_t)(h->dense32.bytes + h->U32.bytes + h->V32.bytes + h->mf_panel[0].bytes + h->mf_vs[0].bytes);
-    I.bytes_total = (int64_t)(h->dense.bytes + h->U.bytes + h->V.bytes + h->blocks.bytes + h->d_active_lr.bytes +
-                              h->mf_panel[1].bytes + h->mf_vs[1].bytes) +
-                    I.bytes_fp32;
+    I.bytes_fp32 = (int64_t)(h->dense32.bytes + h->U32.bytes + h->V32.bytes);
+    int64_t fp64_extra = 0;
+    for (auto &L : h->mf) I.bytes_fp32 += (int64_t)(L.panel[0].bytes + L.vs[0].bytes);
+    for (auto &L : h->mf) fp64_extra += (int64_t)(L.panel[1].bytes + L.vs[1].bytes);
+    I.bytes_total = (int64_t)(h->dense.bytes + h->U.bytes + h->V.bytes + h->blocks.bytes + h->d_active_lr.bytes) +
+                    fp64_extra + I.bytes_fp32;
 }
 
 int arh_hmat_assemble(arh_geom *g, double k, double aca_tol, int max_rank, int storage, arh_hmat **out) {

@@ -138,6 +138,7 @@ void geom_build_index(arh_geom *g) {
     ARH_CHECK(area == (int64_t)N * N, ARH_EINVAL, "partition: blocks do not cover N x N exactly once");
 
     upload(g->d_leaf_start, g->leaf_start, s);
+    upload(g->d_leaf_of, g->leaf_of, s);
     upload(g->d_pcol_ptr, g->pcol_ptr, s);
     upload(g->d_panel_off, g->panel_off, s);
     upload(g->d_pcol_idx, g->pcol_idx, s);

@@ -292,17 +292,17 @@ void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const doubl
     ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
     if (precision == ARH_MIXED) {
         ARH_CHECK(h->storage == ARH_MIXED, ARH_ESTATE, "FP32 product requested but the H-matrix has no FP32 copy");
-        // every non-transposed product takes the MFMA engine, whatever nrhs: a right-hand side
-        // then gets the same bits alone and in any batch
-        if (!transpose && h->g->max_leaf <= mfma_max_leaf()) {
-            hmat_apply_mfma(h, 0, nrhs, x, y, s, cols, ncols, gate);
+        // every product takes the MFMA engine, whatever nrhs: a right-hand side then gets the same
+        // bits alone and in any batch
+        if (h->g->max_leaf <= mfma_max_leaf()) {
+            hmat_apply_mfma(h, transpose, 0, nrhs, x, y, s, cols, ncols, gate);
             return;
         }
         make_generic_fp32(h);
         apply_t<float2>(h, transpose, nrhs, h->dense32.as<float2>(), h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
     } else {
-        if (!transpose && h->g->max_leaf <= mfma_max_leaf()) {
-            hmat_apply_mfma(h, 1, nrhs, x, y, s, cols, ncols, gate);
+        if (h->g->max_leaf <= mfma_max_leaf()) {
+            hmat_apply_mfma(h, transpose, 1, nrhs, x, y, s, cols, ncols, gate);
             return;
         }
         if (!transpose) {

@@ -408,27 +408,36 @@ __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split
 }
 
 // ---------------------------------------------------------------- layout (once per assembly)
-// Stacked V: column c of stack entry (block b at stack row o) <- V_b[p][c] (FP64 row-major).
-template <class T>
-__global__ __launch_bounds__(256) void k_stack_v(const LRBlock *__restrict__ blocks, const VMap *__restrict__ maps,
-                                                 const double2 *__restrict__ V, typename Mfma<T>::T2 *__restrict__ Vs) {
-    const VMap M = maps[blockIdx.x];
+// Stacked factors, column-major per stack: entry (c, p) of stack row o_b + p <- V_b[p][c] (forward,
+// FP64 V row-major) or U_b[c][p] (transposed, FP64 U leaf-segmented: the row's leaf chunk).
+template <class T, bool UT>
+__global__ __launch_bounds__(256) void k_stack(const LRBlock *__restrict__ blocks, const StackMap *__restrict__ maps,
+                                               const double2 *__restrict__ F, const int *__restrict__ leaf_start,
+                                               const int *__restrict__ leaf_of,
+                                               typename Mfma<T>::T2 *__restrict__ Vs) {
+    const StackMap M = maps[blockIdx.x];
     const LRBlock B = blocks[M.blk];
-    const int n = B.ce - B.cs, r = B.rank;
+    const int r = B.rank, n = UT ? B.re - B.rs : B.ce - B.cs;
     for (int64_t e = threadIdx.x; e < (int64_t)r * n; e += blockDim.x) {
         const int c = (int)(e / r), p = (int)(e - (int64_t)c * r);
-        const double2 v = V[B.v_off + (int64_t)p * n + c];
+        double2 v;
+        if (UT) {
+            const int l = leaf_of[B.rs + c];
+            const int ls = leaf_start[l] - B.rs, ms = leaf_start[l + 1] - leaf_start[l];
+            v = F[B.u_off + (int64_t)ls * r + (int64_t)p * ms + (c - ls)];
+        } else {
+            v = F[B.v_off + (int64_t)p * n + c];
+        }
         Vs[M.dst + (int64_t)c * M.rtot + p] = Mfma<T>::make(v.x, v.y);
     }
 }
 
-// Fused panels: panel column c of leaf l <- FP64 column at csrc[col0 + c] (>= 0: dense panel
-// offset, <= -2: U offset -(v + 2), -1: zero column).
+// Fused panels: entry i of panel column c of leaf l <- the FP64 value at csrc[col0 + c].
 template <class T>
 __global__ __launch_bounds__(256) void k_fuse_panel(const int *__restrict__ leaf_start,
                                                     const LeafPanel *__restrict__ lpanel,
-                                                    const int64_t *__restrict__ csrc, const double2 *__restrict__ dense,
-                                                    const double2 *__restrict__ U,
+                                                    const ColSrc *__restrict__ csrc, const double2 *__restrict__ dense,
+                                                    const double2 *__restrict__ U, const double2 *__restrict__ V,
                                                     typename Mfma<T>::T2 *__restrict__ panel) {
     const int l = blockIdx.x;
     const int m = leaf_start[l + 1] - leaf_start[l];
@@ -436,170 +445,208 @@ __global__ __launch_bounds__(256) void k_fuse_panel(const int *__restrict__ leaf
     const int64_t tot = (int64_t)L.ngroups * 4 * m;
     for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
         const int c = (int)(e / m), i = (int)(e - (int64_t)c * m);
-        const int64_t so = csrc[L.col0 + c];
+        const ColSrc cs = csrc[L.col0 + c];
+        const double2 *src = cs.kind == 1 ? dense : cs.kind == 2 ? U : V;
         double2 v = make_double2(0.0, 0.0);
-        if (so >= 0)
-            v = dense[so + i];
-        else if (so <= -2)
-            v = U[-(so + 2) + i];
+        if (cs.kind) v = src[cs.off + (int64_t)i * cs.stride];
         panel[L.off + e] = Mfma<T>::make(v.x, v.y);
     }
 }
 
-// Work lists shared by both precisions (host + device copies).
-static void build_lists(arh_hmat *h) {
+// Work lists of one orientation, shared by both precisions (host + device copies).
+//  forward:    stacks of V by column cluster; panels of row leaves: dense columns, then the U
+//              columns (leaf segment) of the low-rank blocks covering the leaf.
+//  transposed: stacks of U^T by row cluster; panels of column leaves: the transposed dense blocks
+//              of the leaf, then the V columns (restricted to the leaf) of the low-rank blocks
+//              whose column range covers it.
+static void build_lists(arh_hmat *h, int tr) {
     arh_geom *g = h->g;
+    MfmaLayout &M = h->mf[tr];
     hipStream_t s = g->ctx->stream;
     const int N = g->N;
     ARH_CHECK(g->max_leaf <= kPartRows, ARH_EINVAL, "MFMA product supports leaves of at most 64 elements");
-    // V stacks: active low-rank blocks grouped by column cluster, in order of first appearance
+    // stacks: active low-rank blocks grouped by their column (forward) / row (transposed) cluster,
+    // in order of first appearance
+    auto cluster = [&](const LRBlock &B) { return tr ? std::make_pair(B.rs, B.re) : std::make_pair(B.cs, B.ce); };
     std::map<std::pair<int, int>, int> sid;
     std::vector<std::vector<int>> members;
     for (int a : h->active_lr) {
-        const LRBlock &B = h->hblocks[a];
-        auto it = sid.emplace(std::make_pair(B.cs, B.ce), (int)members.size());
+        auto it = sid.emplace(cluster(h->hblocks[a]), (int)members.size());
         if (it.second) members.emplace_back();
         members[it.first->second].push_back(a);
     }
-    h->stacks.clear();
-    std::vector<VMap> maps;
+    M.stacks.clear();
+    std::vector<StackMap> maps;
     std::vector<int64_t> t_row(h->hblocks.size(), -1);
     int64_t voff = 0, trow = N;
     for (auto &mem : members) {
-        const LRBlock &B0 = h->hblocks[mem[0]];
-        VStack S{B0.cs, B0.ce - B0.cs, 0, 0, voff, trow};
+        const auto cl = cluster(h->hblocks[mem[0]]);
+        VStack S{cl.first, cl.second - cl.first, 0, 0, voff, trow};
         for (int a : mem) S.rtot += h->hblocks[a].rank;
         int o = 0;
         for (int a : mem) {
-            maps.push_back(VMap{a, S.rtot, voff + o});
+            maps.push_back(StackMap{a, S.rtot, voff + o});
             t_row[a] = trow + o;
             o += h->hblocks[a].rank;
         }
-        h->stacks.push_back(S);
+        M.stacks.push_back(S);
         voff += (int64_t)S.n * S.rtot;
         trow += S.rtot;
     }
-    h->vs_total = voff;
+    M.vs_total = voff;
     ARH_CHECK(trow - N == h->t_total && trow < INT32_MAX, ARH_EINVAL, "stacked intermediates do not match the ranks");
     // phase-1 units: big stacks first, each list by decreasing column count
     std::vector<int2> big, small;
-    for (int k = 0; k < (int)h->stacks.size(); k++)
-        for (int rt = 0; rt * 32 < h->stacks[k].rtot; rt++) (h->stacks[k].n >= kBigCols ? big : small).push_back(int2{k, rt});
+    for (int k = 0; k < (int)M.stacks.size(); k++)
+        for (int rt = 0; rt * 32 < M.stacks[k].rtot; rt++) (M.stacks[k].n >= kBigCols ? big : small).push_back(int2{k, rt});
     auto by_cols = [&](const int2 &p, const int2 &q) {
-        const int np = h->stacks[p.x].n, nq = h->stacks[q.x].n;
+        const int np = M.stacks[p.x].n, nq = M.stacks[q.x].n;
         return np != nq ? np > nq : (p.x != q.x ? p.x < q.x : p.y < q.y);
     };
     std::sort(big.begin(), big.end(), by_cols);
     std::sort(small.begin(), small.end(), by_cols);
-    h->p1big = (int)big.size();
-    h->p1units = big;
-    h->p1units.insert(h->p1units.end(), small.begin(), small.end());
-    // fused panels: per row leaf, the dense panel columns, then the U columns of its low-rank blocks
-    h->lpanel.assign(g->nleaf, LeafPanel{0, 0, 0});
+    M.p1big = (int)big.size();
+    M.p1units = big;
+    M.p1units.insert(M.p1units.end(), small.begin(), small.end());
+    // fused panels
+    M.lpanel.assign(g->nleaf, LeafPanel{0, 0, 0});
     std::vector<int> psrc;
-    std::vector<int64_t> csrc;
+    std::vector<ColSrc> csrc;
     int64_t poff = 0;
     for (int l = 0; l < g->nleaf; l++) {
-        const int rs = g->leaf_start[l], m = g->leaf_start[l + 1] - rs;
+        const int ls = g->leaf_start[l], m = g->leaf_start[l + 1] - ls;
         const int col0 = (int)psrc.size();
-        const int nd = g->pcol_ptr[l + 1] - g->pcol_ptr[l];
-        for (int c = 0; c < nd; c++) {
-            psrc.push_back(g->pcol_idx[g->pcol_ptr[l] + c]);
-            csrc.push_back(g->panel_off[l] + (int64_t)c * m);
-        }
-        for (int e = g->lrR_ptr[l]; e < g->lrR_ptr[l + 1]; e++) {
-            const int b = g->lrR_blk[e];
-            const LRBlock &B = h->hblocks[b];
-            ARH_CHECK(rs >= B.rs && rs + m <= B.re, ARH_EINVAL, "low-rank block does not cover its row leaf");
-            for (int p = 0; p < B.rank; p++) {
-                psrc.push_back((int)(t_row[b] + p));
-                csrc.push_back(-(B.u_off + (int64_t)(rs - B.rs) * B.rank + (int64_t)p * m) - 2);
+        if (!tr) {
+            const int nd = g->pcol_ptr[l + 1] - g->pcol_ptr[l];
+            for (int c = 0; c < nd; c++) {
+                psrc.push_back(g->pcol_idx[g->pcol_ptr[l] + c]);
+                csrc.push_back(ColSrc{g->panel_off[l] + (int64_t)c * m, 1, 1});
+            }
+            for (int e = g->lrR_ptr[l]; e < g->lrR_ptr[l + 1]; e++) {
+                const int b = g->lrR_blk[e];
+                const LRBlock &B = h->hblocks[b];
+                ARH_CHECK(ls >= B.rs && ls + m <= B.re, ARH_EINVAL, "low-rank block does not cover its row leaf");
+                for (int p = 0; p < B.rank; p++) {
+                    psrc.push_back((int)(t_row[b] + p));
+                    csrc.push_back(ColSrc{B.u_off + (int64_t)(ls - B.rs) * B.rank + (int64_t)p * m, 1, 2});
+                }
+            }
+        } else {
+            for (int e = g->dT_ptr[l]; e < g->dT_ptr[l + 1]; e++) {
+                const int b = g->dT_blk[e];
+                const int rl = g->dblk_leafr[b], pc = g->dblk_col[b];
+                const int rs = g->leaf_start[rl], mr = g->leaf_start[rl + 1] - rs;
+                ARH_CHECK(g->dense[4 * b + 2] == ls && g->dense[4 * b + 3] == ls + m, ARH_EINVAL,
+                          "dense block is not a leaf column");
+                for (int j = 0; j < mr; j++) {
+                    psrc.push_back(rs + j);
+                    csrc.push_back(ColSrc{g->panel_off[rl] + (int64_t)pc * mr + j, mr, 1});
+                }
+            }
+            for (int e = g->lrC_ptr[l]; e < g->lrC_ptr[l + 1]; e++) {
+                const int b = g->lrC_blk[e];
+                const LRBlock &B = h->hblocks[b];
+                ARH_CHECK(ls >= B.cs && ls + m <= B.ce, ARH_EINVAL, "low-rank block does not cover its column leaf");
+                const int n = B.ce - B.cs;
+                for (int p = 0; p < B.rank; p++) {
+                    psrc.push_back((int)(t_row[b] + p));
+                    csrc.push_back(ColSrc{B.v_off + (int64_t)p * n + (ls - B.cs), 1, 3});
+                }
             }
         }
-        while ((psrc.size() - col0) % 4) psrc.push_back(0), csrc.push_back(-1);
+        while ((psrc.size() - col0) % 4) psrc.push_back(0), csrc.push_back(ColSrc{0, 0, 0});
         const int ncol = (int)psrc.size() - col0;
-        h->lpanel[l] = LeafPanel{poff, col0, ncol / 4};
+        M.lpanel[l] = LeafPanel{poff, col0, ncol / 4};
         poff += (int64_t)ncol * m;
     }
-    h->panel_total = poff;
+    M.panel_total = poff;
     for (int v : psrc) ARH_CHECK(v >= 0 && v < N + h->t_total, ARH_EINVAL, "panel source row out of range");
-    // row tasks: leaves with more than tg groups are split evenly; largest tasks first
+    // row tasks: leaves with more than tg groups are split evenly
     int64_t gt = 0;
-    for (auto &L : h->lpanel) gt += L.ngroups;
+    for (auto &L : M.lpanel) gt += L.ngroups;
     int tg = (int)std::max<int64_t>(8 * kWaves, (gt + kTaskSlots - 1) / kTaskSlots);
     tg = (tg + kWaves - 1) / kWaves * kWaves;
-    h->rtasks.clear();
-    h->rsplit.clear();
-    h->nslots = 0;
+    M.rtasks.clear();
+    M.rsplit.clear();
+    M.nslots = 0;
     for (int l = 0; l < g->nleaf; l++) {
-        const int n = h->lpanel[l].ngroups;
+        const int n = M.lpanel[l].ngroups;
         const int nt = std::max(1, (n + tg - 1) / tg);
-        if (nt > 1) h->rsplit.push_back(int4{l, h->nslots, nt, 0});
+        if (nt > 1) M.rsplit.push_back(int4{l, M.nslots, nt, 0});
         for (int t = 0; t < nt; t++)
-            h->rtasks.push_back(RowTask{l, (int)((int64_t)n * t / nt), (int)((int64_t)n * (t + 1) / nt),
-                                        nt > 1 ? h->nslots + t : -1});
-        if (nt > 1) h->nslots += nt;
+            M.rtasks.push_back(RowTask{l, (int)((int64_t)n * t / nt), (int)((int64_t)n * (t + 1) / nt),
+                                       nt > 1 ? M.nslots + t : -1});
+        if (nt > 1) M.nslots += nt;
     }
     // XCD-aware order: workgroups are dispatched round-robin over the kXcds dies, each with its own
     // L2, so die x gets a contiguous run of leaves (in tree order, which neighbouring leaves share
     // most of their low-rank blocks and hence their intermediates rows) with about 1/kXcds of the
-    // groups; padding tasks (leaf -1) even out the runs.
+    // groups; padding tasks (leaf -1) even out the runs. Otherwise: largest tasks first.
     if (kXcdOrder) {
         std::vector<std::vector<RowTask>> run(kXcds);
         int64_t acc = 0;
-        for (const RowTask &T : h->rtasks) {
+        for (const RowTask &T : M.rtasks) {
             const int x = (int)std::min<int64_t>(kXcds - 1, acc * kXcds / std::max<int64_t>(1, gt));
             run[x].push_back(T);
             acc += T.g1 - T.g0;
         }
         size_t len = 0;
         for (auto &r : run) len = std::max(len, r.size());
-        h->rtasks.assign(len * kXcds, RowTask{-1, 0, 0, -1});
+        M.rtasks.assign(len * kXcds, RowTask{-1, 0, 0, -1});
         for (int x = 0; x < kXcds; x++)
-            for (size_t i = 0; i < run[x].size(); i++) h->rtasks[i * kXcds + x] = run[x][i];
+            for (size_t i = 0; i < run[x].size(); i++) M.rtasks[i * kXcds + x] = run[x][i];
     } else {
-        std::stable_sort(h->rtasks.begin(), h->rtasks.end(),
+        std::stable_sort(M.rtasks.begin(), M.rtasks.end(),
                          [](const RowTask &p, const RowTask &q) { return p.g1 - p.g0 > q.g1 - q.g0; });
     }
-    upload(h->d_stacks, h->stacks, s);
-    upload(h->d_p1units, h->p1units, s);
-    upload(h->d_lpanel, h->lpanel, s);
-    upload(h->d_psrc, psrc, s);
-    upload(h->d_rtasks, h->rtasks, s);
-    upload(h->d_rsplit, h->rsplit, s);
-    upload(h->d_vmaps, maps, s);
-    upload(h->d_csrc, csrc, s);
-    h->nvmaps = (int)maps.size();
+    upload(M.d_stacks, M.stacks, s);
+    upload(M.d_p1units, M.p1units, s);
+    upload(M.d_lpanel, M.lpanel, s);
+    upload(M.d_psrc, psrc, s);
+    upload(M.d_rtasks, M.rtasks, s);
+    upload(M.d_rsplit, M.rsplit, s);
+    upload(M.d_maps, maps, s);
+    upload(M.d_csrc, csrc, s);
+    M.nmaps = (int)maps.size();
     ARH_HIP(hipStreamSynchronize(s));  // the host vectors above are freed on return
-    h->mf_lists = true;
+    M.lists = true;
 }
 
-// The operator copy of one precision (0: FP32, 1: FP64) in the MFMA layout.
+// The operator copy of one orientation and precision (0: FP32, 1: FP64) in the MFMA layout.
 template <class T>
-static void build_copy(arh_hmat *h, int pi) {
+static void build_copy(arh_hmat *h, int tr, int pi) {
     typedef typename Mfma<T>::T2 T2;
     arh_geom *g = h->g;
+    MfmaLayout &M = h->mf[tr];
     hipStream_t s = g->ctx->stream;
-    h->mf_vs[pi].alloc(sizeof(T2) * std::max<int64_t>(1, h->vs_total));
-    h->mf_panel[pi].alloc(sizeof(T2) * std::max<int64_t>(1, h->panel_total));
-    if (h->nvmaps)
-        hipLaunchKernelGGL(k_stack_v<T>, dim3((unsigned)h->nvmaps), dim3(256), 0, s, h->blocks.as<LRBlock>(),
-                           h->d_vmaps.as<VMap>(), h->V.as<double2>(), h->mf_vs[pi].as<T2>());
+    M.vs[pi].alloc(sizeof(T2) * std::max<int64_t>(1, M.vs_total));
+    M.panel[pi].alloc(sizeof(T2) * std::max<int64_t>(1, M.panel_total));
+    if (M.nmaps) {
+        if (tr)
+            hipLaunchKernelGGL((k_stack<T, true>), dim3((unsigned)M.nmaps), dim3(256), 0, s, h->blocks.as<LRBlock>(),
+                               M.d_maps.as<StackMap>(), h->U.as<double2>(), g->d_leaf_start.as<int>(),
+                               g->d_leaf_of.as<int>(), M.vs[pi].as<T2>());
+        else
+            hipLaunchKernelGGL((k_stack<T, false>), dim3((unsigned)M.nmaps), dim3(256), 0, s, h->blocks.as<LRBlock>(),
+                               M.d_maps.as<StackMap>(), h->V.as<double2>(), g->d_leaf_start.as<int>(),
+                               g->d_leaf_of.as<int>(), M.vs[pi].as<T2>());
+    }
     hipLaunchKernelGGL(k_fuse_panel<T>, dim3(g->nleaf), dim3(256), 0, s, g->d_leaf_start.as<int>(),
-                       h->d_lpanel.as<LeafPanel>(), h->d_csrc.as<int64_t>(), h->dense.as<double2>(),
-                       h->U.as<double2>(), h->mf_panel[pi].as<T2>());
+                       M.d_lpanel.as<LeafPanel>(), M.d_csrc.as<ColSrc>(), h->dense.as<double2>(), h->U.as<double2>(),
+                       h->V.as<double2>(), M.panel[pi].as<T2>());
     ARH_HIP(hipGetLastError());
     ARH_HIP(hipStreamSynchronize(s));
-    h->mf_built[pi] = true;
+    M.built[pi] = true;
 }
 
-void mfma_prepare(arh_hmat *h, int fp64) {
-    if (!h->mf_lists) build_lists(h);
-    if (h->mf_built[fp64]) return;
+void mfma_prepare(arh_hmat *h, int tr, int fp64) {
+    MfmaLayout &M = h->mf[tr];
+    if (!M.lists) build_lists(h, tr);
+    if (M.built[fp64]) return;
     if (fp64)
-        build_copy<double>(h, 1);
+        build_copy<double>(h, tr, 1);
     else
-        build_copy<float>(h, 0);
+        build_copy<float>(h, tr, 0);
 }
 
 // Widest pass (NT output tiles) that fits the registers with 4 row tiles (leaves over 32 elements).
@@ -609,17 +656,17 @@ constexpr int max_nt_tall() {
 }
 
 template <class T, int NT>
-static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, const int *cols, int j0, const int *gate,
-                      hipStream_t s, int maxrt) {
+static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, double2 *y, const int *cols, int j0,
+                      const int *gate, hipStream_t s, int maxrt) {
     typedef typename Mfma<T>::T2 T2;
     constexpr int pi = sizeof(T) == 8 ? 1 : 0;
     arh_geom *g = h->g;
     Profiler &pf = g->ctx->prof;
     const int N = g->N;
     constexpr int RP = 8 * NT;
-    h->mf_xt[pi].alloc(sizeof(T2) * (size_t)(N + h->t_total) * RP);
-    if (h->nslots) h->mf_part[pi].alloc(sizeof(T) * (size_t)h->nslots * kPartRows * 2 * RP);
-    T2 *xt = h->mf_xt[pi].as<T2>();
+    M.xt[pi].alloc(sizeof(T2) * (size_t)(N + h->t_total) * RP);
+    if (M.nslots) M.part[pi].alloc(sizeof(T) * (size_t)M.nslots * kPartRows * 2 * RP);
+    T2 *xt = M.xt[pi].as<T2>();
     hipLaunchKernelGGL(k_pack_xf<T>, dim3((unsigned)std::min<int64_t>(2048, ((int64_t)N * RP + 255) / 256)), dim3(256),
                        0, s, x, N, R, RP, xt, cols, j0, gate);
     auto timed = [&](int kind, double bytes, double flops, auto &&launch) {
@@ -634,48 +681,48 @@ static void launch_nt(arh_hmat *h, int R, const double2 *x, double2 *y, const in
         pf.pending.push_back(e);
     };
     const double eb = 2.0 * sizeof(T);  // bytes of one complex operator entry
-    const int nu = (int)h->p1units.size(), nb = h->p1big;
+    const int nu = (int)M.p1units.size(), nb = M.p1big;
+    // phase-1 entries: V (forward) or U (transposed); both total sum_b r_b (m_b or n_b)
+    const int64_t e1 = M.vs_total;
     if (nu)
-        timed(pi ? PROF_LR64 : PROF_LR32, eb * h->v_total + 16.0 * R * (N + h->t_total), 8.0 * R * h->v_total, [&] {
+        timed(pi ? PROF_LR64 : PROF_LR32, eb * e1 + 16.0 * R * (N + h->t_total), 8.0 * R * e1, [&] {
             hipLaunchKernelGGL((k_lr_mfma<T, NT>), dim3(nb + (nu - nb + kWaves - 1) / kWaves), dim3(64 * kWaves), 0,
-                               s, h->d_stacks.as<VStack>(), h->d_p1units.as<int2>(), nb, nu, h->mf_vs[pi].as<T2>(),
-                               xt, gate);
+                               s, M.d_stacks.as<VStack>(), M.d_p1units.as<int2>(), nb, nu, M.vs[pi].as<T2>(), xt,
+                               gate);
         });
-    const int64_t ent = g->dense_entries + h->u_total;
-    T *part = h->nslots ? h->mf_part[pi].as<T>() : nullptr;
-    const int ntask = (int)h->rtasks.size(), nsp = (int)h->rsplit.size();
+    const int64_t ent = g->dense_entries + (h->u_total + h->v_total - e1);
+    T *part = M.nslots ? M.part[pi].as<T>() : nullptr;
+    const int ntask = (int)M.rtasks.size(), nsp = (int)M.rsplit.size();
     const double *xd = (const double *)x;
     double *yd = (double *)y;
     timed(pi ? PROF_ROWS64 : PROF_ROWS32, eb * ent + 16.0 * R * (2.0 * N + h->t_total), 8.0 * R * ent, [&] {
         if (maxrt <= 2) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 2>), dim3(ntask), dim3(64 * kWaves), 0, s,
-                               h->d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), h->d_lpanel.as<LeafPanel>(),
-                               h->d_psrc.as<int>(), h->mf_panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0,
-                               gate, part);
+            hipLaunchKernelGGL((k_rows_mfma<T, NT, 2>), dim3(ntask), dim3(64 * kWaves), 0, s, M.d_rtasks.as<RowTask>(),
+                               g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),
+                               M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         } else if constexpr (NT <= max_nt_tall<T>()) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 4>), dim3(ntask), dim3(64 * kWaves), 0, s,
-                               h->d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), h->d_lpanel.as<LeafPanel>(),
-                               h->d_psrc.as<int>(), h->mf_panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0,
-                               gate, part);
+            hipLaunchKernelGGL((k_rows_mfma<T, NT, 4>), dim3(ntask), dim3(64 * kWaves), 0, s, M.d_rtasks.as<RowTask>(),
+                               g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),
+                               M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         } else {
             ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 32 elements");
         }
     });
     if (nsp)
-        hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp), dim3(256), 0, s, h->d_rsplit.as<int4>(),
+        hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp), dim3(256), 0, s, M.d_rsplit.as<int4>(),
                            g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
     ARH_HIP(hipGetLastError());
 }
 
 int mfma_max_leaf() { return kPartRows; }
 
-// y = (M - I/2) x on the MFMA layout of one precision, right-hand sides in passes of up to kMaxRC.
-// With a column list (device array of ncols indices into the nrhs columns of x and y), only those
-// columns are computed; the others of y are left untouched. A non-null gate skips every pass when
-// *gate is 0.
+// y = (M - I/2) x, or its transpose, on the MFMA layout of one precision, right-hand sides in passes
+// of up to kMaxRC. With a column list (device array of ncols indices into the nrhs columns of x
+// and y), only those columns are computed; the others of y are left untouched. A non-null gate
+// skips every pass when *gate is 0.
 template <class T>
-static void apply_t(arh_hmat *h, int R, const double2 *x, double2 *y, hipStream_t s, const int *cols, int ncols,
-                    const int *gate) {
+static void apply_t(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, double2 *y, hipStream_t s, const int *cols,
+                    int ncols, const int *gate) {
     arh_geom *g = h->g;
     const int maxrt = (g->max_leaf + 15) / 16;
     const int nc = cols ? ncols : R;
@@ -684,23 +731,24 @@ static void apply_t(arh_hmat *h, int R, const double2 *x, double2 *y, hipStream_
     for (int j0 = 0; j0 < nc; j0 += maxrc) {
         const int Rc = std::min(maxrc, nc - j0);
         switch ((Rc + 7) / 8) {
-        case 1: launch_nt<T, 1>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
-        case 2: launch_nt<T, 2>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
-        case 3: launch_nt<T, 3>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
-        case 4: launch_nt<T, 4>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
-        default: launch_nt<T, 5>(h, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 1: launch_nt<T, 1>(h, M, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 2: launch_nt<T, 2>(h, M, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 3: launch_nt<T, 3>(h, M, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        case 4: launch_nt<T, 4>(h, M, Rc, x, y, cols, j0, gate, s, maxrt); break;
+        default: launch_nt<T, 5>(h, M, Rc, x, y, cols, j0, gate, s, maxrt); break;
         }
     }
 }
 
-void hmat_apply_mfma(arh_hmat *h, int fp64, int R, const double2 *x, double2 *y, hipStream_t s, const int *cols,
-                     int ncols, const int *gate) {
+void hmat_apply_mfma(arh_hmat *h, int transpose, int fp64, int R, const double2 *x, double2 *y, hipStream_t s,
+                     const int *cols, int ncols, const int *gate) {
     ARH_CHECK(h->g->max_leaf <= mfma_max_leaf(), ARH_EINVAL, "MFMA product supports leaves of at most 64 elements");
-    mfma_prepare(h, fp64);
+    const int tr = transpose ? 1 : 0;
+    mfma_prepare(h, tr, fp64 ? 1 : 0);
     if (fp64)
-        apply_t<double>(h, R, x, y, s, cols, ncols, gate);
+        apply_t<double>(h, h->mf[tr], R, x, y, s, cols, ncols, gate);
     else
-        apply_t<float>(h, R, x, y, s, cols, ncols, gate);
+        apply_t<float>(h, h->mf[tr], R, x, y, s, cols, ncols, gate);
 }
 
 }  // namespace arh
