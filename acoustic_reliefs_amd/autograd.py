@@ -20,6 +20,8 @@ reference's recompute_matrices, adjoint.cpp:104-109, 173-193), trading assembly 
 Heights are the y offsets of the differentiable vertices (the relief's top grid), as in
 DiffBEM.gradient (acoustics3d.cpp:210-213). Everything runs through the C-ABI (capi.py).
 """
+import time
+
 import numpy as np
 import torch
 
@@ -44,6 +46,8 @@ class ReliefBEM:
         self.c0, self.aca_tol, self.tol, self.storage, self.recompute = c0, aca_tol, tol, storage, recompute
         self._saved = None
         self.last_stats = []
+        self.timing = []  # per band: assemble / solve / receivers (forward), adjoint / grad (backward), ms
+        self.log = None   # optional callable(str): a line per band as it completes
 
     def k(self, f):
         return 2 * np.pi * f / self.c0
@@ -59,20 +63,31 @@ class ReliefBEM:
         R, LL = len(self.dirs), len(self.L)
         saved, d_b, d_all = [], [], []
         self.last_stats = []
+        self.timing = []
         for f in self.bands:
             k = self.k(f)
             H = self.g.assemble(k, self.aca_tol, 100, self.storage)
+            info = H.info()
             B = self.g.rhs(k, capi.ARH_RHS_PLANE, self.dirs)
             Bn = B.numpy()
             B.upload(-Bn)
             X = self.ctx.empty((R, self.g.N))
             st = H.solve(B, X, R, tol=self.tol, true_residual=True)
+            t0 = time.perf_counter()
             Y = self.g.receivers(k, self.L, X, nrhs=R)
             d, _, vf = self.ctx.diffusion(Y, LL, R, want_vf=True)
+            t_rec = 1000 * (time.perf_counter() - t0)
             d_all.append(d)
             d_b.append(float(np.mean(d)))
-            saved.append({"k": k, "X": X, "VF": vf, "H": None if self.recompute else H})
+            if not self.recompute:
+                H.compact()  # keep the assembled operator only; the backward rebuilds its product copies
+            saved.append({"k": k, "X": X, "VF": vf, "H": None if self.recompute else H,
+                          "bytes": info["bytes_total"]})
             self.last_stats.append(st)
+            self.timing.append({"band": f, "assemble_ms": info["assemble_ms"], "solve_ms": st["solve_ms"],
+                                "receivers_ms": t_rec, "iters": st["iters"], "hmat_bytes": info["bytes_total"]})
+            if self.log:
+                self.log("forward band %g Hz: %s" % (f, self.timing[-1]))
         self._saved = saved
         return np.array(d_b), np.array(d_all)
 
@@ -88,11 +103,24 @@ class ReliefBEM:
             vf = s["VF"].numpy() * w
             VF = self.ctx.to_device(vf)
             ADJ = self.g.receivers(s["k"], self.L, VF, nrhs=R, transpose=True)
-            H = s["H"] or self.g.assemble(s["k"], self.aca_tol, 100, self.storage)
+            H = s["H"]
+            t_asm = 0.0
+            if H is None:
+                H = self.g.assemble(s["k"], self.aca_tol, 100, self.storage)
+                t_asm = H.info()["assemble_ms"]
             LAM = self.ctx.empty((R, self.g.N))
-            H.solve(ADJ, LAM, R, transpose=True, tol=self.tol, true_residual=True)
+            st = H.solve(ADJ, LAM, R, transpose=True, tol=self.tol, true_residual=True)
+            t0 = time.perf_counter()
             A, Bv, C = self.g.shape_grad(s["k"], self.L, self.diff, R, s["X"], LAM, VF, capi.ARH_RHS_PLANE, self.dirs)
+            t_grad = 1000 * (time.perf_counter() - t0)
             g += A - Bv + C
+            s["H"] = None  # a band's operator is released once its backward is done
+            if b < len(self.timing):
+                self.timing[b].update({"bw_assemble_ms": t_asm, "adjoint_ms": st["solve_ms"],
+                                       "adjoint_iters": st["iters"], "shape_grad_ms": t_grad})
+            if self.log:
+                self.log("backward band %d: assemble %.1f ms, adjoint %.1f ms (%d iters), shape gradient %.1f ms" % (
+                    b, t_asm, st["solve_ms"], st["iters"], t_grad))
         return g
 
 

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("arh_hmat_ranks", c_int, [c_void_p, P_I]),
     ("arh_hmat_dense_block", c_int, [c_void_p, c_int, P_D]),
     ("arh_hmat_factors", c_int, [c_void_p, c_int, P_D, P_D, P_I]),
+    ("arh_hmat_compact", c_int, [c_void_p]),
     ("arh_hmat_destroy", None, [c_void_p]),
     ("arh_hmat_matvec", c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     ("arh_solve", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, ctypes.POINTER(SolveOpts),
@@ -371,6 +372,10 @@ class HMatrix:
         if getattr(self, "h", None) and not _EXITING:
             lib().arh_hmat_destroy(self.h)
             self.h = None
+
+    def compact(self):
+        """Free the derived product copies (FP32 copy, MFMA layouts); the FP64 operator stays."""
+        _check(lib().arh_hmat_compact(self.h), self.geom.ctx.h)
 
     def info(self):
         i = HMatInfo()

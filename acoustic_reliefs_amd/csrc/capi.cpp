@@ -359,6 +359,22 @@ int arh_hmat_factors(const arh_hmat *h, int a, double *U, double *V, int *rank) 
     });
 }
 
+int arh_hmat_compact(arh_hmat *h) {
+    return guard(h ? h->g->ctx : nullptr, [&] {
+        ARH_CHECK(h, ARH_EINVAL, "h is NULL");
+        set_device(h->g->ctx);
+        ARH_HIP(hipStreamSynchronize(h->g->ctx->stream));
+        for (auto &L : h->mf)
+            for (int p = 0; p < 2; p++) {
+                L.panel[p].reset(), L.vs[p].reset(), L.xt[p].reset(), L.part[p].reset();
+                L.built[p] = false;
+            }
+        h->dense32.reset(), h->U32.reset(), h->V32.reset(), h->tbuf.reset();
+        h->generic32 = false;
+        fill_info(h);
+    });
+}
+
 void arh_hmat_destroy(arh_hmat *h) {
     if (!h) return;
     arh_geom *g = h->g;

@@ -86,8 +86,10 @@ static int sguard(F &&f) {
 // (+ FP32 copy), solver vectors, the receiver matrix and per-element records.
 static int64_t footprint(const SMesh &m, int storage, int nrhs, int LL) {
     const bool mixed = storage == ARH_MIXED;
-    const double dense = 16.0 * m.dense_entries * (mixed ? 1.5 : 1.0);
-    const double lowrank = 16.0 * m.aca_scratch * (mixed ? 2.5 : 2.0);
+    // FP64 operator + its FP64 MFMA layout (true residuals) + the FP32 MFMA layout when mixed; the
+    // low-rank part bounded by the ACA scratch (rank <= lim), plus the scratch itself
+    const double dense = 16.0 * m.dense_entries * (mixed ? 2.5 : 2.0);
+    const double lowrank = 16.0 * m.aca_scratch * (mixed ? 3.5 : 3.0);
     const double vec = 16.0 * m.N * (double)nrhs * 14.0;
     const double q = 16.0 * (double)LL * m.N;
     return (int64_t)(dense + lowrank + vec + q + 256.0 * m.N + (64 << 20));

@@ -275,13 +275,16 @@ __global__ void k_zero_cols(const RhsState *__restrict__ S, int N, double2 *__re
         a[o + i] = make_double2(0.0, 0.0);
 }
 
-// x += d (refinement), masked by `active`
+// x += f d (refinement; f = -1 reverts a correction), masked per column
 __global__ void k_axpy_cols(const int *__restrict__ mask, int N, double2 *__restrict__ x,
-                            const double2 *__restrict__ d) {
+                            const double2 *__restrict__ d, double f) {
     const int j = blockIdx.y;
     if (!mask[j]) return;
     const int64_t o = (int64_t)j * N;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) x[o + i] = cadd(x[o + i], d[o + i]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double2 di = d[o + i];
+        x[o + i] = cadd(x[o + i], make_double2(f * di.x, f * di.y));
+    }
 }
 
 // r = b - Ax, per rhs partials of |r|^2 and |b|^2 (true residual)
@@ -344,10 +347,13 @@ int pick_nb(int N) { return std::max(1, std::min(kMaxNB, (N + 2047) / 2048)); }
 
 // BiCGSTAB on operator `prec` (ARH_FP64 / ARH_MIXED) for the rhs with want[j] != 0, from x = 0.
 // Returns per-rhs iterations; fills recursive residual history.
+// stall_window > 0 (inner solves of the refinement only): a right-hand side whose recursive
+// residual has not halved within stall_window iterations is stopped (checked at synchronisations);
+// 0 keeps Eigen's loop control exactly.
 static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 *b, double2 *x,
                      const std::vector<int> &want, const std::vector<double> &tolj, int maxit, const arh_solve_opts &o,
                      arh_solve_stats *st, std::vector<int> &iters, std::vector<double> &err, Work &W, int NB,
-                     bool record_hist) {
+                     bool record_hist, int stall_window = 0) {
     arh_ctx *ctx = h->g->ctx;
     hipStream_t s = ctx->stream;
     const int N = h->g->N;
@@ -403,6 +409,8 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
     // separate host buffers for the two lists: a list is rewritten only after the stream has been
     // synchronised since its last copy
     std::vector<int> ctl(R + 1), ctl_rs(R + 1);
+    std::vector<double> best(err);
+    std::vector<int> best_it(R, 0);
     while (nact > 0) {
         int nc = 0;
         for (int j = 0; j < R; j++)
@@ -434,6 +442,23 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
             for (int k = 1; k < K; k++) ctx->prof.slot_ok[k] = gh[k - 1] > 0;
             ctx->prof.flush(false);
             ctx->prof.slot_ok.clear();
+        }
+        // stalled inner solves stop here (nothing has run on the device since the state was read)
+        if (stall_window > 0) {
+            bool changed = false;
+            for (int j = 0; j < R; j++) {
+                if (!(hs[j].active || hs[j].restart) || hs[j].rhs_sq == 0.0) continue;
+                const double e = std::sqrt(hs[j].rn / hs[j].rhs_sq);
+                if (e < 0.5 * best[j]) {
+                    best[j] = e;
+                    best_it[j] = hs[j].iters;
+                } else if (hs[j].iters - best_it[j] >= stall_window) {
+                    hs[j].active = 0;
+                    hs[j].restart = 0;
+                    changed = true;
+                }
+            }
+            if (changed) ARH_HIP(hipMemcpyAsync(W.S, hs.data(), sizeof(RhsState) * R, hipMemcpyHostToDevice, s));
         }
         // breakdown restarts: r = b - A x, r0 = r for the paused right-hand sides
         int nrs = 0;
@@ -555,7 +580,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
                      err, W, NB, false);
             ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_axpy_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s,
-                               W.mask, N, x, dx.as<double2>());
+                               W.mask, N, x, dx.as<double2>(), 1.0);
             ARH_HIP(hipGetLastError());
             for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
             true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
@@ -564,14 +589,19 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
         }
         err = last_err;
     } else {
-        // mixed: FP32-operator BiCGSTAB inside FP64 true-residual refinement
+        // mixed: FP32-operator BiCGSTAB inside FP64 true-residual refinement. A sweep's correction is
+        // kept for a right-hand side only if it cuts the FP64 true residual; an inner solve that
+        // stalls, or a correction that does not pay, sends the right-hand side to FP64 correction
+        // solves (the FP64 operator, Eigen's loop control), which converge whenever the FP64 path does.
         DBuf rb, dx;
         rb.alloc(sizeof(double2) * (size_t)N * R);
         dx.alloc(sizeof(double2) * (size_t)N * R);
         ARH_HIP(hipMemcpyAsync(rb.p, b, sizeof(double2) * (size_t)N * R, hipMemcpyDeviceToDevice, s));
         ARH_HIP(hipMemsetAsync(x, 0, sizeof(double2) * (size_t)N * R, s));
         std::vector<double> bnorm_rel(R, 1.0);  // ||r|| / ||b|| of the current residual
-        std::vector<int> redo(R, 1);
+        std::vector<int> redo(R, 1), fp64(R, 0);
+        const dim3 ga(std::max(1, std::min(1024, (N + 255) / 256)), R);
+        const int stall = std::max(200, std::min(maxit, 400));
         for (int sweep = 0; sweep < 10; sweep++) {
             // per-rhs inner tolerance relative to its current residual, aiming at half the target
             // (FP32 operator: not below 1e-7); no coupling between right-hand sides
@@ -579,21 +609,50 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             for (int j = 0; j < R; j++)
                 itol[j] = std::min(0.5, std::max(1e-7, 0.5 * tol / std::max(bnorm_rel[j], 1e-300)));
             bicgstab(h, transpose, ARH_MIXED, R, rb.as<double2>(), dx.as<double2>(), redo, itol, maxit, o, st, iters,
-                     err, W, NB, sweep == 0);
+                     err, W, NB, sweep == 0, stall);
             for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
             ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_axpy_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s,
-                               W.mask, N, x, dx.as<double2>());
+            hipLaunchKernelGGL(k_axpy_cols, ga, dim3(256), 0, s, W.mask, N, x, dx.as<double2>(), 1.0);
             ARH_HIP(hipGetLastError());
             true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
             if (st) st->refinements++;
-            int nre = 0;
+            std::vector<int> undo(R, 0);
+            int nundo = 0, nre = 0;
             for (int j = 0; j < R; j++) {
-                bnorm_rel[j] = tres[j];
-                redo[j] = tres[j] > tol ? 1 : 0;
+                if (redo[j] && tres[j] > tol && tres[j] > 0.5 * bnorm_rel[j]) {
+                    fp64[j] = 1;  // the FP32 correction did not pay: finish in FP64
+                    if (tres[j] > bnorm_rel[j]) undo[j] = 1, nundo++;
+                }
+                if (!undo[j]) bnorm_rel[j] = tres[j];
+                redo[j] = (bnorm_rel[j] > tol && !fp64[j]) ? 1 : 0;
                 nre += redo[j];
             }
+            if (nundo) {  // revert corrections that made the residual worse
+                ARH_HIP(hipMemcpyAsync(W.mask, undo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_axpy_cols, ga, dim3(256), 0, s, W.mask, N, x, dx.as<double2>(), -1.0);
+                ARH_HIP(hipGetLastError());
+                true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
+                for (int j = 0; j < R; j++) bnorm_rel[j] = tres[j];
+            }
             if (!nre) break;
+        }
+        // FP64 correction solves for the right-hand sides still above tol
+        for (int sweep = 0; sweep < 8; sweep++) {
+            std::vector<int> f64(R, 0);
+            int nf = 0;
+            for (int j = 0; j < R; j++)
+                if (tres[j] > tol) f64[j] = 1, nf++;
+            if (!nf) break;
+            std::vector<double> ctol(R);
+            for (int j = 0; j < R; j++) ctol[j] = std::min(0.5, 0.5 * tol / std::max(tres[j], 1e-300));
+            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), f64, ctol, maxit, o, st, iters,
+                     err, W, NB, false);
+            for (int j = 0; j < R; j++) iters_tot[j] += f64[j] ? iters[j] : 0;
+            ARH_HIP(hipMemcpyAsync(W.mask, f64.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_axpy_cols, ga, dim3(256), 0, s, W.mask, N, x, dx.as<double2>(), 1.0);
+            ARH_HIP(hipGetLastError());
+            true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
+            if (st) st->refinements++;
         }
         err = tres;
     }

@@ -195,6 +195,70 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
+def run_gradient(args):
+    """Adjoint gradient of loss = mean_b (1 - d_b) (d_b: band mean over the 37 incidences) through the
+    torch.autograd.Function: forward + backward over all bands, mixed precision, with stored H-matrices
+    and with recompute-on-backward; optionally central differences (h = 1e-5 m) on 16 seeded top
+    vertices against the exact (all-dense) operator in FP64 and mixed precision."""
+    import torch
+    from acoustic_reliefs_amd import autograd, capi, meshes
+    P, E, top = meshes.relief_config(size=args.size, grid=args.grid, hmax=args.hmax, seed=0)
+    bands = [float(f) for f in args.freqs.split(",")] if args.freqs else meshes.THIRD_OCTAVE_BANDS[:args.bands]
+    dirs, L = meshes.incidence_directions(37), meshes.semicircle_receivers(5.0, 181)
+    ctx = capi.Context(0)
+    out = {"workload": "%.1f m relief, %dx%d top, N=%d, %d diff vertices, %d bands x 37 plane waves, 181 receivers"
+                       % (args.size, args.grid, args.grid, len(E), len(top), len(bands))}
+    for recompute in (False, True):
+        t_init = time.perf_counter()
+        eng = autograd.ReliefBEM(P, E, top, bands, dirs, L, c0=C0, storage=capi.ARH_MIXED, recompute=recompute,
+                                 ctx=ctx)
+        eng.log = log
+        ctx.synchronize()
+        log("engine (partition, geometry) %.2f s" % (time.perf_counter() - t_init))
+        free0 = torch.cuda.mem_get_info()[0]
+        h = torch.zeros(len(top), dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        d = autograd.diffusion_coefficients(h, eng)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        free1 = torch.cuda.mem_get_info()[0]
+        loss = (1.0 - d).mean()
+        loss.backward()
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        key = "recompute" if recompute else "stored"
+        out[key] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "loss": float(loss),
+                    "device_bytes_held_after_forward": free0 - free1,
+                    "hmat_bytes_per_band": [b["hmat_bytes"] for b in eng.timing],
+                    "per_band": eng.timing}
+        log("gradient %s: forward %.2f s, backward %.2f s, %.2f GB held after the forward" % (
+            key, t1 - t0, t2 - t1, (free0 - free1) / 1e9))
+        del eng, d, loss, h
+        ctx.synchronize()
+    if args.fd:
+        rng = np.random.default_rng(1)
+        pick = rng.choice(len(top), args.fd, replace=False)
+        step = 1e-5
+        for name, storage, tol in (("fp64", capi.ARH_FP64, 1e-10), ("mixed", capi.ARH_MIXED, 1e-10)):
+            eng = autograd.ReliefBEM(P, E, top, bands, dirs, L, c0=C0, storage=storage, tol=tol, ctx=ctx,
+                                     all_dense=True)
+            h = torch.zeros(len(top), dtype=torch.float64, requires_grad=True)
+            (1.0 - autograd.diffusion_coefficients(h, eng)).mean().backward()
+            g = h.grad.numpy()
+            errs = []
+            for t in pick:
+                hp, hm = np.zeros(len(top)), np.zeros(len(top))
+                hp[t], hm[t] = step, -step
+                fd = (np.mean(1.0 - eng.forward(hp)[0]) - np.mean(1.0 - eng.forward(hm)[0])) / (2 * step)
+                errs.append(abs(fd - g[t]) / max(abs(fd), 1e-12))
+            out["fd_" + name] = {"vertices": [int(t) for t in pick], "max_rel_err": float(max(errs)),
+                                 "median_rel_err": float(np.median(errs)), "step_m": step,
+                                 "operator": "exact (every leaf pair dense), BiCGSTAB tol %g" % tol}
+            log("fd %s: max rel err %.2e" % (name, max(errs)))
+            del eng
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +271,11 @@ def main():
     ap.add_argument("--freqs", type=str, default="", help="comma-separated band frequencies (Hz) instead of --bands")
     ap.add_argument("--workers", type=int, default=4, help="server workers (concurrent band solves)")
     ap.add_argument("--sequential", action="store_true", help="solve the bands one after another (no server)")
+    ap.add_argument("--gradient", action="store_true",
+                    help="adjoint-gradient workload instead (forward + backward, stored vs recompute)")
+    ap.add_argument("--fd", type=int, default=0, help="with --gradient: central differences on this many vertices")
+    ap.add_argument("--size", type=float, default=0.6, help="with --gradient: relief width (m)")
+    ap.add_argument("--hmax", type=float, default=0.1, help="with --gradient: maximum height (m)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated timing of the dominant kernel (the roofline figures)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -214,6 +283,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.gradient:
+        run_gradient(args)
         return
     if args.roofline_only:
         from acoustic_reliefs_amd import capi

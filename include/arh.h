@@ -158,6 +158,11 @@ int arh_hmat_dense_block(const arh_hmat *h, int b, double *out);
 /* Factors of low-rank block a as stored (crosses with u, v != 0): *rank = r; U (m x r column-major) and
  * V (r x n row-major), c128 host out, written when non-NULL. */
 int arh_hmat_factors(const arh_hmat *h, int a, double *U, double *V, int *rank);
+/* New (no reference counterpart; for callers that keep many H-matrices, as the gradient does
+ * without recompute_matrices, adjoint.cpp:104-109): free the H-matrix's derived product copies
+ * (FP32 copy, MFMA layouts, work buffers), keeping the assembled FP64 operator. The next product
+ * rebuilds what it needs; results are unchanged. */
+int arh_hmat_compact(arh_hmat *h);
 void arh_hmat_destroy(arh_hmat *h);
 
 /* y = (M - I/2) x or its (non-conjugate) transpose, nrhs vectors, device c128.

@@ -85,3 +85,17 @@ def test_band_solve_mixed_vs_fp64(relief, capi):
     assert np.abs(d32 - d64).max() <= 1e-4
     assert st32["max_true_res"] <= 1e-6 and st64["max_true_res"] <= 1e-6
     assert info32["bytes_fp32"] > 0 and info64["bytes_fp32"] == 0
+
+
+def test_compact_keeps_results(relief, capi):
+    """arh_hmat_compact frees the derived product copies; the rebuilt copies give the same bits."""
+    g, h, k = relief
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((6, g.N)) + 1j * rng.standard_normal((6, g.N))
+    before = [h.matvec_np(X, transpose=t, precision=p) for t in (False, True) for p in (capi.ARH_MIXED, capi.ARH_FP64)]
+    b32 = h.info()["bytes_fp32"]
+    h.compact()
+    assert h.info()["bytes_fp32"] < b32
+    after = [h.matvec_np(X, transpose=t, precision=p) for t in (False, True) for p in (capi.ARH_MIXED, capi.ARH_FP64)]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
