@@ -23,7 +23,9 @@
 namespace arh {
 
 constexpr int kGT = 128;  // threads per workgroup (one touched element each)
-constexpr int kRG = 4;    // rhs per pass
+// rhs per pass: each pass re-evaluates every kernel derivative, so the pass is as wide as the
+// registers allow (the accumulators live in AGPRs; 37 plane waves take 3 passes)
+constexpr int kRG = 13;
 
 __device__ __forceinline__ void FdF(double k, double r, double2 &F, double2 &dF) {
     double sn, cs;
