@@ -105,6 +105,11 @@ SIGNATURES = [
                           ctypes.POINTER(SolveStats)]),
     ("arh_rhs", c_int, [c_void_p, c_double, c_int, c_int, P_D, c_void_p]),
     ("arh_receivers", c_int, [c_void_p, c_double, P_D, c_int, c_int, c_int, c_void_p, c_void_p]),
+    ("arh_qaca_create", c_int, [c_void_p, c_double, P_D, c_int, c_double, c_int, ctypes.POINTER(c_void_p)]),
+    ("arh_qaca_rank", c_int, [c_void_p, P_I]),
+    ("arh_qaca_apply", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    ("arh_qaca_factors", c_int, [c_void_p, P_D, P_D]),
+    ("arh_qaca_destroy", None, [c_void_p]),
     ("arh_diffusion", c_int, [c_void_p, c_int, c_int, c_void_p, P_D, c_void_p, c_void_p]),
     ("arh_shape_grad", c_int, [c_void_p, c_double, P_D, c_int, P_I, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
                                P_D, c_void_p, c_void_p, c_void_p]),
@@ -329,6 +334,10 @@ class Geometry:
                self.ctx.h)
         return out
 
+    def listener_aca(self, k, L, tol, max_rank=5000):
+        """Listener ACA Q ~= U V (compute_ACA_listener, aca.cpp:84-134) built on the device."""
+        return ListenerACA(self, k, L, tol, max_rank)
+
     def shape_grad(self, k, L, diff, nrhs, x, lam, vf, src_kind, src):
         L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 3)
         diff = np.ascontiguousarray(diff, dtype=np.int32)
@@ -359,6 +368,41 @@ class Geometry:
                  "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr,
                  "solve_ms": st.solve_ms, "post_ms": st.post_ms}
         return d, x, y, stats, info.as_dict()
+
+
+class ListenerACA:
+    """Low-rank listener matrix (arh_qaca): Q ~= U V, Q_lj = M(L_l, element j)."""
+
+    def __init__(self, geom, k, L, tol, max_rank=5000):
+        self.geom = geom
+        L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 3)
+        self.LL = len(L)
+        h = c_void_p()
+        _check(lib().arh_qaca_create(geom.h, float(k), _dp(L), self.LL, float(tol), int(max_rank), ctypes.byref(h)),
+               geom.ctx.h)
+        self.h = h
+        r = c_int()
+        _check(lib().arh_qaca_rank(self.h, ctypes.byref(r)))
+        self.rank = r.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and not _EXITING:
+            lib().arh_qaca_destroy(self.h)
+            self.h = None
+
+    def apply(self, x, nrhs=1, transpose=False):
+        """y = U (V x) (x nrhs x N, y nrhs x LL); transpose: y = V^T (U^T x) (x nrhs x LL)."""
+        out = self.geom.ctx.empty((nrhs, self.geom.N if transpose else self.LL))
+        _check(lib().arh_qaca_apply(self.h, int(transpose), int(nrhs), _ptr(x), _ptr(out)), self.geom.ctx.h)
+        return out
+
+    def factors(self):
+        """U (LL x rank), V (rank x N)."""
+        r, LL, N = self.rank, self.LL, self.geom.N
+        U = np.zeros(max(1, LL * r), np.complex128)
+        V = np.zeros(max(1, N * r), np.complex128)
+        _check(lib().arh_qaca_factors(self.h, _dp(U.view(np.float64)), _dp(V.view(np.float64))), self.geom.ctx.h)
+        return U[:LL * r].reshape(r, LL).T, V[:N * r].reshape(r, N)
 
 
 class HMatrix:

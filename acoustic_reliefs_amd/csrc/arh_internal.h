@@ -300,6 +300,17 @@ struct arh_hmat_s {
     arh_hmat_info info{};
 };
 
+// Listener ACA of the LL x N receiver matrix: U (LL x lim, column-major) and V (lim x N,
+// row-major) factors of which the first `rank` crosses are kept; device loop state, partial
+// reductions, used-row flags and the listener coordinates.
+struct arh_qaca_s {
+    arh_geom *g = nullptr;
+    double k = 0, tol = 0;
+    int LL = 0, N = 0, lim = 0, rank = 0;
+    int nbr = 0, nbc = 0;  // blocks over a row (N) and over a column (LL)
+    arh::DBuf L, U, V, used, part, dots, st, t;
+};
+
 namespace arh {
 // geometry.hip
 void geom_build_tri(arh_geom *g);
@@ -327,6 +338,9 @@ void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int tr
                       double2 *y);
 void launch_scale(hipStream_t s, double2 *a, int64_t n, double f);
 void launch_diffusion(hipStream_t s, int LL, int nrhs, const double2 *y, double *d, double *u, double2 *vf);
+// qaca.hip
+void qaca_build(arh_qaca *q, const double *L_host);
+void qaca_apply(arh_qaca *q, int transpose, int nrhs, const double2 *x, double2 *y);
 // solver.hip
 void solve(arh_hmat *h, int transpose, int nrhs, const double2 *b, double2 *x, const arh_solve_opts &o,
            arh_solve_stats *st);

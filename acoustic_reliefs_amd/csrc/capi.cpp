@@ -431,6 +431,61 @@ int arh_receivers(arh_geom *g, double k, const double *L, int LL, int transpose,
     });
 }
 
+int arh_qaca_create(arh_geom *g, double k, const double *L, int LL, double tol, int max_rank, arh_qaca **out) {
+    return guard(g ? g->ctx : nullptr, [&] {
+        ARH_CHECK(g && L && out, ARH_EINVAL, "NULL argument");
+        ARH_CHECK(LL >= 1, ARH_EINVAL, "LL must be >= 1");
+        ARH_CHECK(max_rank >= 0, ARH_EINVAL, "max_rank must be >= 0");
+        ARH_CHECK(std::isfinite(k) && k > 0, ARH_EINVAL, "k must be finite and > 0");
+        set_device(g->ctx);
+        auto q = std::make_unique<arh_qaca>();
+        q->g = g, q->k = k, q->tol = tol, q->LL = LL, q->N = g->N;
+        q->lim = std::min(max_rank, std::min(LL - 1, g->N - 1));
+        qaca_build(q.get(), L);
+        *out = q.release();
+        g->refs++;
+    });
+}
+
+int arh_qaca_rank(const arh_qaca *q, int *rank) {
+    return guard(nullptr, [&] {
+        ARH_CHECK(q && rank, ARH_EINVAL, "NULL argument");
+        *rank = q->rank;
+    });
+}
+
+int arh_qaca_apply(arh_qaca *q, int transpose, int nrhs, const double *x, double *y) {
+    return guard(q ? q->g->ctx : nullptr, [&] {
+        ARH_CHECK(q && x && y, ARH_EINVAL, "NULL argument");
+        ARH_CHECK(x != y, ARH_EINVAL, "x and y must not alias");
+        ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
+        set_device(q->g->ctx);
+        qaca_apply(q, transpose, nrhs, (const double2 *)x, (double2 *)y);
+        ARH_HIP(hipStreamSynchronize(q->g->ctx->stream));
+    });
+}
+
+int arh_qaca_factors(const arh_qaca *q, double *U, double *V) {
+    return guard(q ? q->g->ctx : nullptr, [&] {
+        ARH_CHECK(q, ARH_EINVAL, "NULL argument");
+        set_device(q->g->ctx);
+        hipStream_t s = q->g->ctx->stream;
+        const size_t r = (size_t)q->rank;
+        if (U && r) ARH_HIP(hipMemcpyAsync(U, q->U.p, sizeof(double2) * r * q->LL, hipMemcpyDeviceToHost, s));
+        if (V && r) ARH_HIP(hipMemcpyAsync(V, q->V.p, sizeof(double2) * r * q->N, hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+void arh_qaca_destroy(arh_qaca *q) {
+    if (!q) return;
+    arh_geom *g = q->g;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    delete q;
+    geom_release(g);
+}
+
 int arh_diffusion(arh_ctx *ctx, int LL, int nrhs, const double *y, double *d, double *u, double *vf) {
     return guard(ctx, [&] {
         ARH_CHECK(ctx && y && d, ARH_EINVAL, "NULL argument");

@@ -29,6 +29,7 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;  // constants.h:12
 constexpr int kApproxMaxRank = 100;              // APPROX_ACA_MAX_K, constants.h:16
+constexpr int kQAcaMaxRank = 5000;               // Q_ACA_MAX_K, constants.h:17
 
 void check(int rc, arh_ctx *ctx) {
     if (rc != ARH_OK) throw std::runtime_error(std::string("acoustics3d (gfx950): ") + arh_last_error(ctx));
@@ -173,6 +174,7 @@ struct Forward {
     std::vector<cd> y;     // LL
     std::shared_ptr<DevBuf> x;
     arh_hmat *h = nullptr;
+    arh_qaca *q = nullptr;  // listener ACA kept for the backward (stored matrices)
 };
 
 }  // namespace
@@ -300,11 +302,15 @@ public:
         std::vector<cd> y(LL);
         {
             py::gil_scoped_release nogil;
-            Mesh &m = active_mesh(mesh_->P);
+            Mesh &m = hmat_mesh(mesh_->P);
             const double k = 2 * kPi * f / c0_;
-            auto x = solve_forward(m, k, nullptr);
+            auto x = solve_forward(m, k, nullptr, false);
             DevBuf yd(ctx_->c, sizeof(cd) * LL);
-            check(arh_receivers(m.g, k, L.data(), LL, 0, 1, x->d(), yd.d()), ctx_->c);
+            // acoustics3d.cpp:275-276: y = U (V x) from the listener ACA at Q_ACA_tol
+            arh_qaca *q = listener_aca(m, k, L);
+            const int rc = arh_qaca_apply(q, 0, 1, x->d(), yd.d());
+            arh_qaca_destroy(q);
+            check(rc, ctx_->c);
             yd.down(y.data(), sizeof(cd) * LL);
         }
         py::array_t<cd> out(LL);
@@ -321,9 +327,9 @@ public:
         std::vector<cd> xs(N);
         {
             py::gil_scoped_release nogil;
-            Mesh &m = active_mesh(mesh_->P);
+            Mesh &m = hmat_mesh(mesh_->P);
             const double k = 2 * kPi * f / c0_;
-            auto x = solve_forward(m, k, nullptr);
+            auto x = solve_forward(m, k, nullptr, false);
             x->down(xs.data(), sizeof(cd) * N);
         }
         py::array_t<cd> out(N);
@@ -402,18 +408,32 @@ private:
         return *m;
     }
 
-    arh_solve_opts opts() const {
+    // The H-matrix mesh whatever use_actual says (pvals and surface_vals always take it,
+    // acoustics3d.cpp:240-316).
+    Mesh &hmat_mesh(const std::vector<double> &P) {
+        check(arh_geom_set_vertices(mesh_->g, P.data()), ctx_->c);
+        return *mesh_;
+    }
+
+    // compute_ACA_listener(..., Q_ACA_tol, Q_ACA_MAX_K) (aca.cpp:84-134) for listeners L at k
+    arh_qaca *listener_aca(Mesh &m, double k, const std::vector<double> &L) {
+        arh_qaca *q = nullptr;
+        check(arh_qaca_create(m.g, k, L.data(), (int)L.size() / 3, q_tol_, kQAcaMaxRank, &q), ctx_->c);
+        return q;
+    }
+
+    arh_solve_opts opts(bool actual) const {
         arh_solve_opts o{};
-        o.tol = use_actual ? 1e-13 : solver_tol_;
+        o.tol = actual ? 1e-13 : solver_tol_;
         o.maxit = 0;
-        o.true_residual = use_actual ? 1 : 0;
+        o.true_residual = actual ? 1 : 0;
         o.fail_on_noconv = 0;
         o.hist_cap = 0;
         return o;
     }
 
     // assemble at k and solve (M - I/2) x = -b for the point source (adjoint.cpp:63-97)
-    std::shared_ptr<DevBuf> solve_forward(Mesh &m, double k, arh_hmat **keep) {
+    std::shared_ptr<DevBuf> solve_forward(Mesh &m, double k, arh_hmat **keep, bool actual) {
         arh_hmat *h = nullptr;
         check(arh_hmat_assemble(m.g, k, approx_tol_, kApproxMaxRank, ARH_FP64, &h), ctx_->c);
         const int N = m.N();
@@ -424,7 +444,7 @@ private:
         b.down(hb.data(), sizeof(cd) * N);
         for (auto &v : hb) v = -v;
         b.up(hb.data(), sizeof(cd) * N);
-        arh_solve_opts o = opts();
+        arh_solve_opts o = opts(actual);
         arh_solve_stats st{};
         int rc = arh_solve(h, 0, 1, b.d(), x->d(), &o, &st);
         if (rc != ARH_OK) {
@@ -447,19 +467,42 @@ private:
         const int N = m.N(), LL = (int)L_.size() / 3;
         auto freqs = get_frequencies(F, n_freqs_);
         std::vector<Forward> fw;
+        fw.reserve(freqs.size());
+        // kept operators are released on every exit, error paths included
+        struct Release {
+            std::vector<Forward> &fw;
+            ~Release() {
+                for (auto &s : fw) {
+                    if (s.h) arh_hmat_destroy(s.h);
+                    if (s.q) arh_qaca_destroy(s.q);
+                }
+            }
+        } release{fw};
         std::vector<double> yt(LL, 0.0);
         for (double f : freqs) {
-            Forward st;
+            fw.emplace_back();
+            Forward &st = fw.back();
             st.k = 2 * kPi * f / c0_;
             arh_hmat *h = nullptr;
-            st.x = solve_forward(m, st.k, (forward_only || recompute_) ? nullptr : &h);
+            st.x = solve_forward(m, st.k, (forward_only || recompute_) ? nullptr : &h, use_actual);
             st.h = h;
             DevBuf yd(ctx_->c, sizeof(cd) * LL);
-            check(arh_receivers(m.g, st.k, L_.data(), LL, 0, 1, st.x->d(), yd.d()), ctx_->c);
+            if (use_actual) {
+                // the exact listener matrix (adjoint.cpp:75-77)
+                check(arh_receivers(m.g, st.k, L_.data(), LL, 0, 1, st.x->d(), yd.d()), ctx_->c);
+            } else {
+                // y = U (V x) (adjoint.cpp:101-102); kept for the backward unless recomputing
+                arh_qaca *q = listener_aca(m, st.k, L_);
+                const int rc = arh_qaca_apply(q, 0, 1, st.x->d(), yd.d());
+                if (rc != ARH_OK || forward_only || recompute_)
+                    arh_qaca_destroy(q);
+                else
+                    st.q = q;
+                check(rc, ctx_->c);
+            }
             st.y.resize(LL);
             yd.down(st.y.data(), sizeof(cd) * LL);
             for (int l = 0; l < LL; l++) yt[l] += std::abs(st.y[l]);
-            fw.push_back(std::move(st));
         }
         std::vector<double> u;
         const double c = diffusion(yt, u);
@@ -474,10 +517,19 @@ private:
             std::vector<cd> vf(LL);
             for (int l = 0; l < LL; l++) vf[l] = u[l] * std::conj(st.y[l]) / std::abs(st.y[l]);
             vfd.up(vf.data(), sizeof(cd) * LL);
-            check(arh_receivers(m.g, st.k, L_.data(), LL, 1, 1, vfd.d(), adj.d()), ctx_->c);
+            if (use_actual) {
+                check(arh_receivers(m.g, st.k, L_.data(), LL, 1, 1, vfd.d(), adj.d()), ctx_->c);
+            } else {
+                // adj_b = V^T (U^T vf) (adjoint.cpp:177-178 recomputed, :195 stored)
+                arh_qaca *q = st.q ? st.q : listener_aca(m, st.k, L_);
+                st.q = nullptr;
+                const int rc = arh_qaca_apply(q, 1, 1, vfd.d(), adj.d());
+                arh_qaca_destroy(q);
+                check(rc, ctx_->c);
+            }
             arh_hmat *h = st.h;
             if (!h) check(arh_hmat_assemble(m.g, st.k, approx_tol_, kApproxMaxRank, ARH_FP64, &h), ctx_->c);
-            arh_solve_opts o = opts();
+            arh_solve_opts o = opts(use_actual);
             arh_solve_stats ss{};
             int rc = arh_solve(h, 1, 1, adj.d(), lam.d(), &o, &ss);
             arh_hmat_destroy(h);
@@ -570,7 +622,17 @@ py::array_t<cd> sphere(const std::string &path, double freq, int LL, double lrad
             L[3 * l] = lrad * std::cos(t);
             L[3 * l + 2] = -lrad * std::sin(t);
         }
-        check(arh_receivers(m->g, k, L.data(), LL, 0, 1, x.d(), yd.d()), ctx->c);
+        if (actual) {
+            // the exact listener matrix (sphere.h:65-67)
+            check(arh_receivers(m->g, k, L.data(), LL, 0, 1, x.d(), yd.d()), ctx->c);
+        } else {
+            // listener ACA at tols (sphere.h:80-81)
+            arh_qaca *q = nullptr;
+            check(arh_qaca_create(m->g, k, L.data(), LL, tols, kQAcaMaxRank, &q), ctx->c);
+            const int rc2 = arh_qaca_apply(q, 0, 1, x.d(), yd.d());
+            arh_qaca_destroy(q);
+            check(rc2, ctx->c);
+        }
         yd.down(y.data(), sizeof(cd) * LL);
     }
     py::array_t<cd> out(LL);

@@ -1,0 +1,328 @@
+// Listener ACA on gfx950: one partially pivoted ACA over the whole LL x N receiver matrix
+// Q_lj = entry(listener l, element j) (compute_ACA_listener, aca.cpp:84-134), then y = U (V x) and
+// its transpose V^T (U^T x) (adjoint.cpp:101-102, 177-178).
+//
+// Unlike the block ACA (one workgroup per block), this is a single large block, so every stage of a
+// cross is a grid-wide kernel: residual row (N entries) with block-partial first-max, pivot, row
+// normalisation, residual column (LL entries) with masked first-max, the non-conjugated dot products
+// with the stored crosses, and a one-workgroup finish that applies the stopping rule. All reductions
+// are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
+// checks), so the host launches a few crosses at a time and synchronises once per batch.
+//
+// Semantics kept from aca.cpp:84-134: start at row 0 (used = {0}); V row = v / v_j with no zero-pivot
+// guard (aca.cpp:104); error estimate with |u|^2 |v|^2 + 2 sum |u_p^T u| |v v_p^T|; stop when
+// sqrt(|u|^2 |v|^2 / err) <= eps discarding the current cross; crosses are kept even when zero;
+// iteration limit min(max_rank, LL - 1, N - 1).
+#include <algorithm>
+
+#include "arh_internal.h"
+#include "device_common.h"
+
+namespace arh {
+
+constexpr int kQT = 256;  // threads per block
+
+struct QAcaState {
+    double2 piv;
+    double err;
+    int i, j, it, done;
+    int lim, pad0;
+};
+
+// block first-max of (value, index); ties to the lower index; value -1 = none
+__device__ __forceinline__ void block_argmax_q(double &v, int &idx, double *sv, int *si) {
+    wave_argmax(v, idx);
+    const int w = threadIdx.x / ARH_WAVE, lane = threadIdx.x % ARH_WAVE;
+    __syncthreads();
+    if (lane == 0) sv[w] = v, si[w] = idx;
+    __syncthreads();
+    double bv = sv[0];
+    int bi = si[0];
+    for (int q = 1; q < kQT / ARH_WAVE; q++)
+        if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) bv = sv[q], bi = si[q];
+    v = bv;
+    idx = bi;
+}
+
+__device__ __forceinline__ double block_sum_q(double v, double *sv) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / ARH_WAVE, lane = threadIdx.x % ARH_WAVE;
+    __syncthreads();
+    if (lane == 0) sv[w] = v;
+    __syncthreads();
+    double s = sv[0];
+    for (int q = 1; q < kQT / ARH_WAVE; q++) s += sv[q];
+    return s;
+}
+
+// residual row i into V row `it` (unnormalised) + block partial first-max of |v|^2
+__global__ __launch_bounds__(kQT) void k_qa_row(const QAcaState *__restrict__ S, const double *__restrict__ L,
+                                               const Tri *__restrict__ tri, double k, int LL, int N,
+                                               const double2 *__restrict__ U, double2 *__restrict__ V,
+                                               double *__restrict__ pmax) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    __shared__ int si[kQT / ARH_WAVE];
+    const QAcaState st = *S;
+    if (st.done) return;
+    const int i = st.i, it = st.it;
+    const double px = L[3 * i], py = L[3 * i + 1], pz = L[3 * i + 2];
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    const int c = blockIdx.x * kQT + threadIdx.x;
+    if (c < N) {
+        double2 v = bem_entry(k, px, py, pz, tri[c]);
+        for (int p = 0; p < it; p++) {
+            const double2 up = U[(int64_t)p * LL + i], vp = V[(int64_t)p * N + c];
+            v.x -= up.x * vp.x - up.y * vp.y;
+            v.y -= up.x * vp.y + up.y * vp.x;
+        }
+        V[(int64_t)it * N + c] = v;
+        const double a = cnorm2(v);
+        best = a == a ? a : -1.0, bi = c;  // NaN never wins
+    }
+    block_argmax_q(best, bi, sv, si);
+    if (threadIdx.x == 0) pmax[2 * blockIdx.x] = best, pmax[2 * blockIdx.x + 1] = (double)bi;
+}
+
+// pivot column j = first max over the block partials (block order = index order); piv = v_j
+__global__ void k_qa_pivot(QAcaState *__restrict__ S, const double *__restrict__ pmax, int nb, int N,
+                           const double2 *__restrict__ V) {
+    if (threadIdx.x != 0 || S->done) return;
+    double bv = pmax[0];
+    int bj = (int)pmax[1];
+    for (int b = 1; b < nb; b++)
+        if (pmax[2 * b] > bv) bv = pmax[2 * b], bj = (int)pmax[2 * b + 1];
+    S->j = bj;
+    S->piv = V[(int64_t)S->it * N + bj];
+}
+
+// V row `it` /= v_j (no zero-pivot guard, aca.cpp:104) + block partials of |v|^2
+__global__ __launch_bounds__(kQT) void k_qa_norm(const QAcaState *__restrict__ S, int N, double2 *__restrict__ V,
+                                                double *__restrict__ pvn) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    const QAcaState st = *S;
+    if (st.done) return;
+    const int c = blockIdx.x * kQT + threadIdx.x;
+    double a = 0.0;
+    if (c < N) {
+        const double2 v = cdiv(V[(int64_t)st.it * N + c], st.piv);
+        V[(int64_t)st.it * N + c] = v;
+        a = cnorm2(v);
+    }
+    a = block_sum_q(a, sv);
+    if (threadIdx.x == 0) pvn[blockIdx.x] = a;
+}
+
+// residual column j into U column `it` + block partials of |u|^2 and of the masked first-max
+__global__ __launch_bounds__(kQT) void k_qa_col(const QAcaState *__restrict__ S, const double *__restrict__ L,
+                                               const Tri *__restrict__ tri, double k, int LL, int N,
+                                               double2 *__restrict__ U, const double2 *__restrict__ V,
+                                               const unsigned char *__restrict__ used, double *__restrict__ pcol) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    __shared__ int si[kQT / ARH_WAVE];
+    const QAcaState st = *S;
+    if (st.done) return;
+    const int j = st.j, it = st.it;
+    const Tri tj = tri[j];
+    const int l = blockIdx.x * kQT + threadIdx.x;
+    double a = 0.0, best = -1.0;
+    int bi = 0x7fffffff;
+    if (l < LL) {
+        double2 u = bem_entry(k, L[3 * l], L[3 * l + 1], L[3 * l + 2], tj);
+        for (int p = 0; p < it; p++) {
+            const double2 up = U[(int64_t)p * LL + l], vp = V[(int64_t)p * N + j];
+            u.x -= up.x * vp.x - up.y * vp.y;
+            u.y -= up.x * vp.y + up.y * vp.x;
+        }
+        U[(int64_t)it * LL + l] = u;
+        a = cnorm2(u);
+        if (!used[l]) best = a == a ? a : -1.0, bi = l;
+    }
+    a = block_sum_q(a, sv);
+    block_argmax_q(best, bi, sv, si);
+    if (threadIdx.x == 0) pcol[3 * blockIdx.x] = a, pcol[3 * blockIdx.x + 1] = best, pcol[3 * blockIdx.x + 2] = (double)bi;
+}
+
+// |u^T u_p| |v v_p^T| for the stored crosses p < it (non-conjugated, aca.cpp:64-69 / :114-116);
+// block p of the grid takes cross p
+__global__ __launch_bounds__(kQT) void k_qa_dots(const QAcaState *__restrict__ S, int LL, int N,
+                                                const double2 *__restrict__ U, const double2 *__restrict__ V,
+                                                double *__restrict__ dots) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    const QAcaState st = *S;
+    const int p = blockIdx.x;
+    if (st.done || p >= st.it) return;
+    double2 su = make_double2(0.0, 0.0), sq = make_double2(0.0, 0.0);
+    const double2 *ui = U + (int64_t)st.it * LL, *up = U + (int64_t)p * LL;
+    const double2 *vi = V + (int64_t)st.it * N, *vp = V + (int64_t)p * N;
+    for (int l = threadIdx.x; l < LL; l += kQT) cfma(su, ui[l], up[l]);
+    for (int c = threadIdx.x; c < N; c += kQT) cfma(sq, vi[c], vp[c]);
+    su.x = block_sum_q(su.x, sv);
+    su.y = block_sum_q(su.y, sv);
+    sq.x = block_sum_q(sq.x, sv);
+    sq.y = block_sum_q(sq.y, sv);
+    if (threadIdx.x == 0) dots[p] = cabs_(su) * cabs_(sq);
+}
+
+// error estimate, stopping rule, next row (one thread)
+__global__ void k_qa_finish(QAcaState *__restrict__ S, double eps, const double *__restrict__ pvn, int nbr,
+                            const double *__restrict__ pcol, int nbc, const double *__restrict__ dots,
+                            unsigned char *__restrict__ used) {
+    if (threadIdx.x != 0) return;
+    QAcaState st = *S;
+    if (st.done) return;
+    double vn = 0.0, un = 0.0, d = 0.0;
+    for (int b = 0; b < nbr; b++) vn += pvn[b];
+    double bv = -1.0;
+    int bi = 0;
+    for (int b = 0; b < nbc; b++) {
+        un += pcol[3 * b];
+        if (pcol[3 * b + 1] > bv) bv = pcol[3 * b + 1], bi = (int)pcol[3 * b + 2];
+    }
+    for (int p = 0; p < st.it; p++) d += dots[p];
+    st.err += un * vn + 2.0 * d;
+    if (sqrt(un * vn / st.err) <= eps) {
+        st.done = 1;  // current cross discarded: rank = it
+    } else {
+        st.i = bv < 0.0 ? 0 : bi;
+        used[st.i] = 1;
+        st.it++;
+        if (st.it >= st.lim) st.done = 1;
+    }
+    *S = st;
+}
+
+// products: t = V x (rank x nrhs, one block per (p, j)); y = U t
+__global__ __launch_bounds__(kQT) void k_qa_vx(int N, int r, const double2 *__restrict__ V, const double2 *__restrict__ x,
+                                              double2 *__restrict__ t) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    const int p = blockIdx.x, j = blockIdx.y;
+    double2 s = make_double2(0.0, 0.0);
+    for (int c = threadIdx.x; c < N; c += kQT) {
+        const double2 a = V[(int64_t)p * N + c], b = x[(int64_t)j * N + c];
+        s.x += a.x * b.x - a.y * b.y;
+        s.y += a.x * b.y + a.y * b.x;
+    }
+    s.x = block_sum_q(s.x, sv);
+    s.y = block_sum_q(s.y, sv);
+    if (threadIdx.x == 0) t[(int64_t)j * r + p] = s;
+}
+
+__global__ void k_qa_ut(int LL, int r, int nrhs, const double2 *__restrict__ U, const double2 *__restrict__ t,
+                        double2 *__restrict__ y) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)LL * nrhs) return;
+    const int j = (int)(e / LL), l = (int)(e % LL);
+    double2 s = make_double2(0.0, 0.0);
+    for (int p = 0; p < r; p++) {
+        const double2 a = U[(int64_t)p * LL + l], b = t[(int64_t)j * r + p];
+        s.x += a.x * b.x - a.y * b.y;
+        s.y += a.x * b.y + a.y * b.x;
+    }
+    y[e] = s;
+}
+
+// transposed: s = U^T x (rank x nrhs, one block per (p, j)); y = V^T s
+__global__ __launch_bounds__(kQT) void k_qa_utx(int LL, int r, const double2 *__restrict__ U,
+                                               const double2 *__restrict__ x, double2 *__restrict__ t) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    const int p = blockIdx.x, j = blockIdx.y;
+    double2 s = make_double2(0.0, 0.0);
+    for (int l = threadIdx.x; l < LL; l += kQT) {
+        const double2 a = U[(int64_t)p * LL + l], b = x[(int64_t)j * LL + l];
+        s.x += a.x * b.x - a.y * b.y;
+        s.y += a.x * b.y + a.y * b.x;
+    }
+    s.x = block_sum_q(s.x, sv);
+    s.y = block_sum_q(s.y, sv);
+    if (threadIdx.x == 0) t[(int64_t)j * r + p] = s;
+}
+
+__global__ void k_qa_vts(int N, int r, int nrhs, const double2 *__restrict__ V, const double2 *__restrict__ t,
+                         double2 *__restrict__ y) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)N * nrhs) return;
+    const int j = (int)(e / N), c = (int)(e % N);
+    double2 s = make_double2(0.0, 0.0);
+    for (int p = 0; p < r; p++) {
+        const double2 a = V[(int64_t)p * N + c], b = t[(int64_t)j * r + p];
+        s.x += a.x * b.x - a.y * b.y;
+        s.y += a.x * b.y + a.y * b.x;
+    }
+    y[e] = s;
+}
+
+// build the factors (host loop over batches of crosses)
+void qaca_build(arh_qaca *q, const double *L_host) {
+    arh_geom *g = q->g;
+    hipStream_t s = g->ctx->stream;
+    const int LL = q->LL, N = q->N, lim = q->lim;
+    upload(q->L, std::vector<double>(L_host, L_host + 3 * (size_t)LL), s);
+    q->U.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)LL * lim));
+    q->V.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)lim * N));
+    q->nbr = (N + kQT - 1) / kQT;
+    q->nbc = (LL + kQT - 1) / kQT;
+    q->part.alloc(sizeof(double) * (2 * (size_t)q->nbr + q->nbr + 3 * (size_t)q->nbc + 16));
+    q->dots.alloc(sizeof(double) * std::max(1, lim));
+    std::vector<unsigned char> used(LL, 0);
+    used[0] = 1;
+    upload(q->used, used, s);
+    QAcaState st0{};
+    st0.piv = make_double2(1.0, 0.0);
+    st0.i = 0, st0.j = 0, st0.it = 0, st0.done = lim <= 0 ? 1 : 0, st0.lim = lim;
+    q->st.alloc(sizeof(QAcaState));
+    ARH_HIP(hipMemcpyAsync(q->st.p, &st0, sizeof(QAcaState), hipMemcpyHostToDevice, s));
+    double *pmax = q->part.as<double>(), *pvn = pmax + 2 * q->nbr, *pcol = pvn + q->nbr;
+    QAcaState *S = q->st.as<QAcaState>();
+    const Tri *tri = g->tri.as<Tri>();
+    const double2 *Uc = q->U.as<double2>();
+    const double2 *Vc = q->V.as<double2>();
+    int launched = 0;  // crosses launched so far (an upper bound of it on the device)
+    QAcaState hs = st0;
+    while (!hs.done) {
+        const int batch = std::min(8, lim - launched);
+        for (int b = 0; b < batch; b++) {
+            const int itb = launched + b;  // the cross index if none of the previous ones stopped
+            hipLaunchKernelGGL(k_qa_row, dim3(q->nbr), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N, Uc,
+                               q->V.as<double2>(), pmax);
+            hipLaunchKernelGGL(k_qa_pivot, dim3(1), dim3(64), 0, s, S, pmax, q->nbr, N, Vc);
+            hipLaunchKernelGGL(k_qa_norm, dim3(q->nbr), dim3(kQT), 0, s, S, N, q->V.as<double2>(), pvn);
+            hipLaunchKernelGGL(k_qa_col, dim3(q->nbc), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N,
+                               q->U.as<double2>(), Vc, q->used.as<unsigned char>(), pcol);
+            if (itb > 0)
+                hipLaunchKernelGGL(k_qa_dots, dim3(itb), dim3(kQT), 0, s, S, LL, N, Uc, Vc, q->dots.as<double>());
+            hipLaunchKernelGGL(k_qa_finish, dim3(1), dim3(64), 0, s, S, q->tol, pvn, q->nbr, pcol, q->nbc,
+                               q->dots.as<double>(), q->used.as<unsigned char>());
+        }
+        ARH_HIP(hipGetLastError());
+        launched += batch;
+        ARH_HIP(hipMemcpyAsync(&hs, q->st.p, sizeof(QAcaState), hipMemcpyDeviceToHost, s));
+        ARH_HIP(hipStreamSynchronize(s));
+        ARH_CHECK(hs.done || launched < lim, ARH_EINVAL, "internal: listener ACA ran past its limit");
+    }
+    q->rank = hs.it;
+}
+
+void qaca_apply(arh_qaca *q, int transpose, int nrhs, const double2 *x, double2 *y) {
+    hipStream_t s = q->g->ctx->stream;
+    const int r = q->rank;
+    const int64_t ny = (int64_t)(transpose ? q->N : q->LL) * nrhs;
+    if (r == 0) {
+        ARH_HIP(hipMemsetAsync(y, 0, sizeof(double2) * ny, s));
+        return;
+    }
+    q->t.alloc(sizeof(double2) * (size_t)r * nrhs);
+    if (!transpose) {
+        hipLaunchKernelGGL(k_qa_vx, dim3(r, nrhs), dim3(kQT), 0, s, q->N, r, q->V.as<double2>(), x, q->t.as<double2>());
+        hipLaunchKernelGGL(k_qa_ut, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, s, q->LL, r, nrhs,
+                           q->U.as<double2>(), q->t.as<double2>(), y);
+    } else {
+        hipLaunchKernelGGL(k_qa_utx, dim3(r, nrhs), dim3(kQT), 0, s, q->LL, r, q->U.as<double2>(), x,
+                           q->t.as<double2>());
+        hipLaunchKernelGGL(k_qa_vts, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, s, q->N, r, nrhs,
+                           q->V.as<double2>(), q->t.as<double2>(), y);
+    }
+    ARH_HIP(hipGetLastError());
+}
+
+}  // namespace arh

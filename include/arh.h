@@ -50,6 +50,7 @@ typedef struct arh_ctx_s arh_ctx;
 typedef struct arh_geom_s arh_geom;
 typedef struct arh_hmat_s arh_hmat;
 typedef struct arh_server_s arh_server;
+typedef struct arh_qaca_s arh_qaca;
 
 typedef struct {
     double tol;        /* relative residual tolerance (BiCGSTAB::setTolerance, adjoint.cpp:93) */
@@ -191,6 +192,23 @@ int arh_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs, doubl
  * with x nrhs x LL (the adjoint right-hand side, adjoint.cpp:198). */
 int arh_receivers(arh_geom *g, double k, const double *L, int LL, int transpose, int nrhs, const double *x,
                   double *y);
+
+/* Listener ACA (compute_ACA_listener, aca.cpp:84-134): partially pivoted cross approximation
+ * Q ~= U V of the whole LL x N listener matrix Q_lj = M(L_l, element j), started at row 0, rows
+ * divided by the pivot without a zero guard, stopped when sqrt(|u|^2 |v|^2 / err) <= tol (the
+ * current cross discarded) or after min(max_rank, LL - 1, N - 1) crosses (the reference passes
+ * Q_ACA_MAX_K = 5000, constants.h:17). Built on the device at create; keeps a reference to g.
+ * L: host LL x 3. The reference's uses: acoustics3d.cpp:275 and adjoint.cpp:101 (y = U (V x)),
+ * adjoint.cpp:177-178 (V^T (U^T vf)), sphere.h:80 (tol = the solver tolerance). */
+int arh_qaca_create(arh_geom *g, double k, const double *L, int LL, double tol, int max_rank, arh_qaca **out);
+int arh_qaca_rank(const arh_qaca *q, int *rank);
+/* y (device c128) = U (V x) for x nrhs x N, y nrhs x LL; transpose = 1: y (nrhs x N) =
+ * V^T (U^T x) for x nrhs x LL. */
+int arh_qaca_apply(arh_qaca *q, int transpose, int nrhs, const double *x, double *y);
+/* Host copies of the kept factors: U (LL x rank, column-major c128), V (rank x N, row-major c128);
+ * either may be NULL. */
+int arh_qaca_factors(const arh_qaca *q, double *U, double *V);
+void arh_qaca_destroy(arh_qaca *q);
 
 /* Diffusion coefficient per rhs from y (device c128, nrhs x LL) with n_freqs = 1:
  * yt = |y|, d = ((sum L)^2 - sum L^2) / ((LL-1) sum L^2), L = yt^2 (adjoint.cpp:112-127, 388-397).

@@ -386,11 +386,13 @@ def sphere_scattered(k, a, r, cos_theta, nterms=None):
 
 # ----------------------------------------------------------------- DiffBEM-equivalent band pipeline
 def band_value_grad(oc, ks, L, diff=None, src=(0.0, 100.0, 0.0), aca_tol=1e-3, solver_tol=1e-3, want_grad=False,
-                    use_actual=False, src_kind=0, directions=None):
-    """bem3d::gradient for one band (adjoint.cpp:24-242) with the exact listener matrix.
+                    use_actual=False, src_kind=0, directions=None, q_aca_tol=None):
+    """bem3d::gradient for one band (adjoint.cpp:24-242).
 
     ks: wavenumbers of the band's frequencies. src_kind 0: point source src; 1: plane wave
-    src (a direction). Returns d, or (d, dd/dh) when want_grad.
+    src (a direction). The listener matrix is the listener ACA U V at q_aca_tol (adjoint.cpp:101,
+    177) when q_aca_tol is given and use_actual is off, else the exact matrix (adjoint.cpp:75).
+    Returns d, or (d, dd/dh) when want_grad.
     """
     states = []
     yt = np.zeros(len(L))
@@ -403,7 +405,11 @@ def band_value_grad(oc, ks, L, diff=None, src=(0.0, 100.0, 0.0), aca_tol=1e-3, s
             A = None
             oc.assemble(k, aca_tol, 100)
             x, _, _, _ = oc.bicgstab(-b, solver_tol)
-        Q = oc.listener_matrix(k, L)
+        if q_aca_tol is not None and not use_actual:
+            U, V = oc.aca_listener(k, L, q_aca_tol)
+            Q = U @ V
+        else:
+            Q = oc.listener_matrix(k, L)
         y = Q @ x
         yt += np.abs(y)
         states.append((k, A, Q, x, y))

@@ -48,7 +48,7 @@ def test_values_vs_oracle(bem, oc, oracle):
     d, Ps, Es, top = bem
     vals = d.values()
     L = d.get_listeners()
-    ref = [oracle.band_value_grad(oc, [2 * np.pi * F / 343.0], L, aca_tol=1e-7, solver_tol=1e-9)
+    ref = [oracle.band_value_grad(oc, [2 * np.pi * F / 343.0], L, aca_tol=1e-7, solver_tol=1e-9, q_aca_tol=1e-7)
            for F in (500.0, 1000.0, 2000.0)]
     print("values", vals, "oracle", ref)
     assert np.abs(np.array(vals) - np.array(ref)).max() < 1e-6
@@ -64,7 +64,7 @@ def test_value_x_vs_oracle(bem, oc, oracle):
     P[top, 1] += x
     oc.set_vertices(P)
     ref = np.mean([oracle.band_value_grad(oc, [2 * np.pi * F / 343.0], d.get_listeners(), aca_tol=1e-7,
-                                          solver_tol=1e-9) for F in (500.0, 1000.0, 2000.0)])
+                                          solver_tol=1e-9, q_aca_tol=1e-7) for F in (500.0, 1000.0, 2000.0)])
     oc.set_vertices(Ps)
     assert abs(v - ref) < 1e-6
 
@@ -78,10 +78,19 @@ def test_pvals_surface_vals(bem, oc, oracle):
     xs = d.surface_vals(f)
     assert xs.dtype == np.complex128 and rel(xs, xo) < 1e-6
     Lc = np.array([[5.0 * np.cos(t), 5.0 * np.sin(t), 0.0] for t in np.linspace(0, np.pi, 19)])
+    # pvals: y = U (V x) from the listener ACA at Q_ACA_tol = 1e-7 (acoustics3d.cpp:275)
     p = d.pvals(f, Lc)
-    assert rel(p, oc.listener_matrix(k, Lc) @ xo) < 1e-6
+    U, V = oc.aca_listener(k, Lc, 1e-7)
+    assert rel(p, U @ (V @ xo)) < 1e-6 and rel(p, oc.listener_matrix(k, Lc) @ xo) < 1e-5
     p0 = d.pvals(f)
-    assert p0.shape == (1297,) and rel(p0, oc.listener_matrix(k, d.get_listeners()) @ xo) < 1e-6
+    U, V = oc.aca_listener(k, d.get_listeners(), 1e-7)
+    assert p0.shape == (1297,) and rel(p0, U @ (V @ xo)) < 1e-6
+    # pvals and surface_vals take the H-matrix path whatever use_actual says (acoustics3d.cpp:240-316)
+    d.use_actual = True
+    try:
+        assert rel(d.pvals(f, Lc), p) < 1e-12 and rel(d.surface_vals(f), xs) < 1e-12
+    finally:
+        d.use_actual = False
 
 
 def test_use_actual_dense(bem, oc, oracle):

@@ -63,7 +63,7 @@ def test_diffbem_gradient_vs_oracle_pipeline(setup, oracle, capi):
     vo, go = 0.0, np.zeros(len(top))
     for F in bands:
         dv, gv = oracle.band_value_grad(oc, [2 * np.pi * F / 343.0], L, top, aca_tol=1e-8, solver_tol=1e-10,
-                                        want_grad=True)
+                                        want_grad=True, q_aca_tol=1e-8)
         vo += dv
         go += gv
     oc.set_vertices(P)

@@ -128,6 +128,38 @@ def test_aca_error_bound_and_svd(oracle, sphere_mesh):
                 assert err >= 0.99 * sv[ranks[blk]] / np.linalg.norm(sv) - 1e-15
 
 
+def test_listener_aca_pinned(oracle, small_relief):
+    """Listener ACA (aca.cpp:84-134) vs the exact listener matrix: error bound, optimal-rank floor,
+    exact interpolation of row 0 (the start row) and of every pivot column, the rank limit
+    min(max_rank, LL - 1, N - 1)."""
+    P, E, _ = small_relief
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    L = oracle.listener_points(50.0, 15.0)
+    k = 2 * np.pi * 2000.0 / 343.0
+    Q = oc.listener_matrix(k, L)
+    sv = np.linalg.svd(Q, compute_uv=False)
+    for tol in (1e-3, 1e-6):
+        U, V = oc.aca_listener(k, L, tol)
+        r = U.shape[1]
+        assert 0 < r < min(Q.shape) and V.shape == (r, Q.shape[1])
+        err = np.linalg.norm(Q - U @ V) / np.linalg.norm(Q)
+        assert err < 30 * tol, (tol, err)
+        assert err >= 0.99 * np.sqrt(np.sum(sv[r:] ** 2)) / np.linalg.norm(sv) - 1e-15
+        # the approximant reproduces the start row exactly (row 0, aca.cpp:92)
+        assert np.linalg.norm((Q - U @ V)[0]) <= 1e-12 * np.linalg.norm(Q[0])
+        # V rows are divided by their largest entry (the pivot): a unit entry at the pivot column,
+        # nothing larger, and the approximant reproduces the pivot columns exactly
+        piv = [int(np.argmin(np.abs(V[p] - 1.0))) for p in range(r)]
+        for p, j in enumerate(piv):
+            assert abs(V[p, j] - 1.0) < 1e-14 and np.abs(V[p]).max() <= 1.0 + 1e-14
+        assert np.linalg.norm((Q - U @ V)[:, piv]) <= 1e-10 * np.linalg.norm(Q[:, piv])
+    U, V = oc.aca_listener(k, L, 1e-12, max_rank=3)
+    assert U.shape[1] == 3
+    U, V = oc.aca_listener(k, L[:2], 1e-12)
+    assert U.shape[1] == 1
+
+
 def test_hmatvec_vs_dense(oracle, sphere_mesh):
     P, E = sphere_mesh
     Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
