@@ -69,7 +69,7 @@ class ServerOpts(ctypes.Structure):
 class RequestInfo(ctypes.Structure):
     _fields_ = [("status", c_int), ("nrhs", c_int), ("t_submit", c_double), ("t_start", c_double),
                 ("t_done", c_double), ("footprint", ctypes.c_int64), ("iters", c_int), ("max_true_res", c_double),
-                ("assemble_ms", c_double), ("solve_ms", c_double), ("post_ms", c_double)]
+                ("assemble_ms", c_double), ("solve_ms", c_double), ("post_ms", c_double), ("geom_ms", c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -84,6 +84,8 @@ SIGNATURES = [
     ("arh_ctx_synchronize", c_int, [c_void_p]),
     ("arh_ctx_profile", c_int, [c_void_p, c_int]),
     ("arh_ctx_get_profile", c_int, [c_void_p, ctypes.POINTER(Profile)]),
+    ("arh_device_trim", c_int, [c_int, ctypes.POINTER(c_size_t)]),
+    ("arh_device_cached", c_int, [c_int, ctypes.POINTER(c_size_t)]),
     ("arh_malloc", c_int, [c_void_p, c_size_t, ctypes.POINTER(c_void_p)]),
     ("arh_free", c_int, [c_void_p, c_void_p]),
     ("arh_upload", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
@@ -167,6 +169,20 @@ def _check(rc, ctx=None):
     if rc != ARH_OK:
         msg = lib().arh_last_error(ctx)
         raise RuntimeError("libarh %s: %s" % (ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+
+def device_trim(device=0):
+    """Return the library's cached device blocks to HIP (arh_device_trim); bytes freed."""
+    f = c_size_t()
+    _check(lib().arh_device_trim(int(device), ctypes.byref(f)))
+    return f.value
+
+
+def device_cached(device=0):
+    """Bytes of freed device blocks the library holds for reuse (arh_device_cached)."""
+    b = c_size_t()
+    _check(lib().arh_device_cached(int(device), ctypes.byref(b)))
+    return b.value
 
 
 def partition(C, leaf, factor):

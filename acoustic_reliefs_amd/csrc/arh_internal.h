@@ -29,34 +29,44 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::arh::Error((code), (msg));          \
     } while (0)
 
-// Owning device buffer.
+// Device memory cache (pool.cpp). dev_alloc rounds n up to the block it hands out; the block is
+// ordered after its previous owner's last use when the streams differ. tl_stream is the stream of
+// the context the calling thread is working for (set on entry to every C-ABI call with a context).
+extern thread_local hipStream_t tl_stream;
+void *dev_alloc(size_t &n, hipStream_t s);
+void *dev_alloc(size_t &n, hipStream_t s, int *device);
+void dev_free(void *p, size_t n, hipStream_t s, int device);
+size_t dev_trim(int device);    // frees every cached block of the device; returns the bytes
+size_t dev_cached(int device);  // bytes held in the cache
+
+// Owning device buffer, from the device memory cache; it belongs to the stream it was allocated
+// for (the calling context's).
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    hipStream_t s = nullptr;
+    int dev = 0;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes), s(o.s), dev(o.dev) { o.p = nullptr, o.bytes = 0; }
     DBuf &operator=(DBuf &&o) noexcept {
         reset();
-        p = o.p, bytes = o.bytes;
+        p = o.p, bytes = o.bytes, s = o.s, dev = o.dev;
         o.p = nullptr, o.bytes = 0;
         return *this;
     }
     ~DBuf() { reset(); }
     void reset() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p, bytes, s, dev);
         p = nullptr, bytes = 0;
     }
     void alloc(size_t n) {
         if (n <= bytes && p) return;
         reset();
         if (n == 0) n = 16;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw Error(ARH_ENOMEM, "hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
-        }
+        s = tl_stream;
+        p = dev_alloc(n, s, &dev);
         bytes = n;
     }
     template <class T>

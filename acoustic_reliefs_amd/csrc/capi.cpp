@@ -38,7 +38,12 @@ static int guard(arh_ctx *ctx, F &&f) {
     }
 }
 
-static void set_device(arh_ctx *ctx) { ARH_HIP(hipSetDevice(ctx->device)); }
+// Every call working for a context: its device is current and its stream owns the buffers the
+// call allocates.
+static void set_device(arh_ctx *ctx) {
+    ARH_HIP(hipSetDevice(ctx->device));
+    tl_stream = ctx->stream;
+}
 
 void *arh_ctx_s::pinned(size_t bytes) {
     if (bytes > hostpin_bytes) {
@@ -79,6 +84,7 @@ static void ctx_release(arh_ctx *ctx) {
     if (ctx->hostpin.p) (void)hipHostFree(ctx->hostpin.p), ctx->hostpin.p = nullptr;
     ctx->red.reset();
     ctx->scal.reset();
+    if (tl_stream == ctx->stream || tl_stream == ctx->own) tl_stream = nullptr;
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
 }
@@ -105,7 +111,14 @@ const char *arh_last_error(const arh_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int arh_ctx_set_stream(arh_ctx *ctx, void *hip_stream) {
     return guard(ctx, [&] {
         ARH_CHECK(ctx, ARH_EINVAL, "ctx is NULL");
+        // buffers belong to the stream they were allocated for: switch before any geometry exists
+        ARH_CHECK(ctx->refs == 1, ARH_ESTATE, "the stream can only be set while the context has no geometry");
+        set_device(ctx);
+        ARH_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->red.reset();
+        ctx->scal.reset();
         ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+        tl_stream = ctx->stream;
     });
 }
 
@@ -132,6 +145,27 @@ int arh_ctx_get_profile(arh_ctx *ctx, arh_profile *out) {
         set_device(ctx);
         ctx->prof.flush(true);
         *out = ctx->prof.acc;
+    });
+}
+
+int arh_device_trim(int device, size_t *freed_bytes) {
+    return guard(nullptr, [&] {
+        int n = 0;
+        ARH_HIP(hipGetDeviceCount(&n));
+        ARH_CHECK(device >= 0 && device < n, ARH_EINVAL, "no such device");
+        int cur = 0;
+        ARH_HIP(hipGetDevice(&cur));
+        ARH_HIP(hipSetDevice(device));
+        const size_t f = dev_trim(device);
+        ARH_HIP(hipSetDevice(cur));
+        if (freed_bytes) *freed_bytes = f;
+    });
+}
+
+int arh_device_cached(int device, size_t *bytes) {
+    return guard(nullptr, [&] {
+        ARH_CHECK(bytes, ARH_EINVAL, "NULL argument");
+        *bytes = dev_cached(device);
     });
 }
 
@@ -523,6 +557,7 @@ int arh_band_solve(arh_geom *g, double k, double aca_tol, int max_rank, int stor
     rc = guard(g->ctx, [&] {
         ARH_CHECK(vecs && L && opts && d, ARH_EINVAL, "NULL argument");
         ARH_CHECK(LL >= 2, ARH_EINVAL, "LL must be >= 2");
+        set_device(g->ctx);
         hipStream_t s = g->ctx->stream;
         const int N = g->N;
         DBuf vb, bb, xb, yb, Ld, dd;

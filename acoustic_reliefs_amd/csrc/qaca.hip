@@ -5,8 +5,8 @@
 // Unlike the block ACA (one workgroup per block), this is a single large block, so every stage of a
 // cross is a grid-wide kernel: residual row (N entries) with block-partial first-max, pivot, row
 // normalisation, residual column (LL entries) with masked first-max, the non-conjugated dot products
-// with the stored crosses, and a one-workgroup finish that applies the stopping rule. All reductions
-// are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
+// with the stored crosses (chunk partials over a 2-D grid), and a one-workgroup finish that applies
+// the stopping rule. All reductions are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
 // checks), so the host launches a few crosses at a time and synchronises once per batch.
 //
 // Semantics kept from aca.cpp:84-134: start at row 0 (used = {0}); V row = v / v_j with no zero-pivot
@@ -84,16 +84,23 @@ __global__ __launch_bounds__(kQT) void k_qa_row(const QAcaState *__restrict__ S,
     if (threadIdx.x == 0) pmax[2 * blockIdx.x] = best, pmax[2 * blockIdx.x + 1] = (double)bi;
 }
 
-// pivot column j = first max over the block partials (block order = index order); piv = v_j
-__global__ void k_qa_pivot(QAcaState *__restrict__ S, const double *__restrict__ pmax, int nb, int N,
-                           const double2 *__restrict__ V) {
-    if (threadIdx.x != 0 || S->done) return;
-    double bv = pmax[0];
-    int bj = (int)pmax[1];
-    for (int b = 1; b < nb; b++)
-        if (pmax[2 * b] > bv) bv = pmax[2 * b], bj = (int)pmax[2 * b + 1];
-    S->j = bj;
-    S->piv = V[(int64_t)S->it * N + bj];
+// pivot column j = first max over the block partials (one wavefront; ties to the lower index, so
+// the result is the sequential first maximum); piv = v_j
+__global__ __launch_bounds__(ARH_WAVE) void k_qa_pivot(QAcaState *__restrict__ S, const double *__restrict__ pmax,
+                                                      int nb, int N, const double2 *__restrict__ V) {
+    if (S->done) return;
+    double bv = -1.0;
+    int bj = 0x7fffffff;
+    for (int b = threadIdx.x; b < nb; b += ARH_WAVE) {
+        const double v = pmax[2 * b];
+        const int j = (int)pmax[2 * b + 1];
+        if (v > bv || (v == bv && j < bj)) bv = v, bj = j;
+    }
+    wave_argmax(bv, bj);
+    if (threadIdx.x == 0) {
+        S->j = bj;
+        S->piv = V[(int64_t)S->it * N + bj];
+    }
 }
 
 // V row `it` /= v_j (no zero-pivot guard, aca.cpp:104) + block partials of |v|^2
@@ -143,43 +150,66 @@ __global__ __launch_bounds__(kQT) void k_qa_col(const QAcaState *__restrict__ S,
     if (threadIdx.x == 0) pcol[3 * blockIdx.x] = a, pcol[3 * blockIdx.x + 1] = best, pcol[3 * blockIdx.x + 2] = (double)bi;
 }
 
-// |u^T u_p| |v v_p^T| for the stored crosses p < it (non-conjugated, aca.cpp:64-69 / :114-116);
-// block p of the grid takes cross p
-__global__ __launch_bounds__(kQT) void k_qa_dots(const QAcaState *__restrict__ S, int LL, int N,
+// Chunk partials of the non-conjugated products u^T u_p and v v_p^T with the stored crosses p < it
+// (aca.cpp:64-69 / :114-116): block (p, c) sums chunk c of kQC entries, chunks [0, ncu) over U's
+// LL rows and [ncu, ncu + ncv) over V's N columns; pu[p * ncu + c], pv[p * ncv + c'].
+constexpr int kQC = 8 * kQT;
+__global__ __launch_bounds__(kQT) void k_qa_dots(const QAcaState *__restrict__ S, int LL, int N, int ncu, int ncv,
                                                 const double2 *__restrict__ U, const double2 *__restrict__ V,
-                                                double *__restrict__ dots) {
+                                                double2 *__restrict__ pu, double2 *__restrict__ pv) {
     __shared__ double sv[kQT / ARH_WAVE];
     const QAcaState st = *S;
-    const int p = blockIdx.x;
+    const int p = blockIdx.x, c = blockIdx.y;
     if (st.done || p >= st.it) return;
-    double2 su = make_double2(0.0, 0.0), sq = make_double2(0.0, 0.0);
-    const double2 *ui = U + (int64_t)st.it * LL, *up = U + (int64_t)p * LL;
-    const double2 *vi = V + (int64_t)st.it * N, *vp = V + (int64_t)p * N;
-    for (int l = threadIdx.x; l < LL; l += kQT) cfma(su, ui[l], up[l]);
-    for (int c = threadIdx.x; c < N; c += kQT) cfma(sq, vi[c], vp[c]);
-    su.x = block_sum_q(su.x, sv);
-    su.y = block_sum_q(su.y, sv);
-    sq.x = block_sum_q(sq.x, sv);
-    sq.y = block_sum_q(sq.y, sv);
-    if (threadIdx.x == 0) dots[p] = cabs_(su) * cabs_(sq);
+    const bool isu = c < ncu;
+    const int n = isu ? LL : N, c0 = (isu ? c : c - ncu) * kQC;
+    const double2 *a = isu ? U + (int64_t)st.it * LL : V + (int64_t)st.it * N;
+    const double2 *b = isu ? U + (int64_t)p * LL : V + (int64_t)p * N;
+    double2 acc = make_double2(0.0, 0.0);
+    const int e1 = min(n, c0 + kQC);
+    for (int e = c0 + threadIdx.x; e < e1; e += kQT) cfma(acc, a[e], b[e]);
+    acc.x = block_sum_q(acc.x, sv);
+    acc.y = block_sum_q(acc.y, sv);
+    if (threadIdx.x == 0) {
+        if (isu)
+            pu[(int64_t)p * ncu + c] = acc;
+        else
+            pv[(int64_t)p * ncv + (c - ncu)] = acc;
+    }
 }
 
-// error estimate, stopping rule, next row (one thread)
-__global__ void k_qa_finish(QAcaState *__restrict__ S, double eps, const double *__restrict__ pvn, int nbr,
-                            const double *__restrict__ pcol, int nbc, const double *__restrict__ dots,
-                            unsigned char *__restrict__ used) {
-    if (threadIdx.x != 0) return;
-    QAcaState st = *S;
-    if (st.done) return;
-    double vn = 0.0, un = 0.0, d = 0.0;
-    for (int b = 0; b < nbr; b++) vn += pvn[b];
-    double bv = -1.0;
-    int bi = 0;
-    for (int b = 0; b < nbc; b++) {
+// error estimate, stopping rule, next row (one workgroup; fixed-order reductions)
+__global__ __launch_bounds__(kQT) void k_qa_finish(QAcaState *__restrict__ S, double eps,
+                                                  const double *__restrict__ pvn, int nbr,
+                                                  const double *__restrict__ pcol, int nbc,
+                                                  const double2 *__restrict__ pu, int ncu,
+                                                  const double2 *__restrict__ pv, int ncv,
+                                                  unsigned char *__restrict__ used) {
+    __shared__ double sv[kQT / ARH_WAVE];
+    __shared__ int si[kQT / ARH_WAVE];
+    const QAcaState st0 = *S;
+    if (st0.done) return;
+    double vn = 0.0, un = 0.0, d = 0.0, bv = -1.0;
+    int bi = 0x7fffffff;
+    for (int b = threadIdx.x; b < nbr; b += kQT) vn += pvn[b];
+    for (int b = threadIdx.x; b < nbc; b += kQT) {
         un += pcol[3 * b];
-        if (pcol[3 * b + 1] > bv) bv = pcol[3 * b + 1], bi = (int)pcol[3 * b + 2];
+        const double v = pcol[3 * b + 1];
+        const int l = (int)pcol[3 * b + 2];
+        if (v > bv || (v == bv && l < bi)) bv = v, bi = l;
     }
-    for (int p = 0; p < st.it; p++) d += dots[p];
+    for (int p = threadIdx.x; p < st0.it; p += kQT) {
+        double2 su = make_double2(0.0, 0.0), sq = make_double2(0.0, 0.0);
+        for (int c = 0; c < ncu; c++) su.x += pu[(int64_t)p * ncu + c].x, su.y += pu[(int64_t)p * ncu + c].y;
+        for (int c = 0; c < ncv; c++) sq.x += pv[(int64_t)p * ncv + c].x, sq.y += pv[(int64_t)p * ncv + c].y;
+        d += cabs_(su) * cabs_(sq);
+    }
+    vn = block_sum_q(vn, sv);
+    un = block_sum_q(un, sv);
+    d = block_sum_q(d, sv);
+    block_argmax_q(bv, bi, sv, si);
+    if (threadIdx.x != 0) return;
+    QAcaState st = st0;
     st.err += un * vn + 2.0 * d;
     if (sqrt(un * vn / st.err) <= eps) {
         st.done = 1;  // current cross discarded: rank = it
@@ -257,13 +287,15 @@ void qaca_build(arh_qaca *q, const double *L_host) {
     arh_geom *g = q->g;
     hipStream_t s = g->ctx->stream;
     const int LL = q->LL, N = q->N, lim = q->lim;
-    upload(q->L, std::vector<double>(L_host, L_host + 3 * (size_t)LL), s);
+    const std::vector<double> hL(L_host, L_host + 3 * (size_t)LL);  // alive until the stream is synchronised
+    upload(q->L, hL, s);
     q->U.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)LL * lim));
     q->V.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)lim * N));
     q->nbr = (N + kQT - 1) / kQT;
     q->nbc = (LL + kQT - 1) / kQT;
     q->part.alloc(sizeof(double) * (2 * (size_t)q->nbr + q->nbr + 3 * (size_t)q->nbc + 16));
-    q->dots.alloc(sizeof(double) * std::max(1, lim));
+    const int ncu = (LL + kQC - 1) / kQC, ncv = (N + kQC - 1) / kQC;
+    q->dots.alloc(sizeof(double2) * (size_t)std::max(1, lim) * (ncu + ncv));
     std::vector<unsigned char> used(LL, 0);
     used[0] = 1;
     upload(q->used, used, s);
@@ -277,22 +309,26 @@ void qaca_build(arh_qaca *q, const double *L_host) {
     const Tri *tri = g->tri.as<Tri>();
     const double2 *Uc = q->U.as<double2>();
     const double2 *Vc = q->V.as<double2>();
+    double2 *pu = q->dots.as<double2>(), *pv = pu + (size_t)std::max(1, lim) * ncu;
     int launched = 0;  // crosses launched so far (an upper bound of it on the device)
     QAcaState hs = st0;
     while (!hs.done) {
-        const int batch = std::min(8, lim - launched);
+        // a few crosses per synchronisation at first, more once the rank grows (launches after the
+        // stop return at once)
+        const int batch = std::min(std::max(8, std::min(32, launched / 4)), lim - launched);
         for (int b = 0; b < batch; b++) {
             const int itb = launched + b;  // the cross index if none of the previous ones stopped
             hipLaunchKernelGGL(k_qa_row, dim3(q->nbr), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N, Uc,
                                q->V.as<double2>(), pmax);
-            hipLaunchKernelGGL(k_qa_pivot, dim3(1), dim3(64), 0, s, S, pmax, q->nbr, N, Vc);
+            hipLaunchKernelGGL(k_qa_pivot, dim3(1), dim3(ARH_WAVE), 0, s, S, pmax, q->nbr, N, Vc);
             hipLaunchKernelGGL(k_qa_norm, dim3(q->nbr), dim3(kQT), 0, s, S, N, q->V.as<double2>(), pvn);
             hipLaunchKernelGGL(k_qa_col, dim3(q->nbc), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N,
                                q->U.as<double2>(), Vc, q->used.as<unsigned char>(), pcol);
             if (itb > 0)
-                hipLaunchKernelGGL(k_qa_dots, dim3(itb), dim3(kQT), 0, s, S, LL, N, Uc, Vc, q->dots.as<double>());
-            hipLaunchKernelGGL(k_qa_finish, dim3(1), dim3(64), 0, s, S, q->tol, pvn, q->nbr, pcol, q->nbc,
-                               q->dots.as<double>(), q->used.as<unsigned char>());
+                hipLaunchKernelGGL(k_qa_dots, dim3(itb, ncu + ncv), dim3(kQT), 0, s, S, LL, N, ncu, ncv, Uc, Vc, pu,
+                                   pv);
+            hipLaunchKernelGGL(k_qa_finish, dim3(1), dim3(kQT), 0, s, S, q->tol, pvn, q->nbr, pcol, q->nbc, pu, ncu,
+                               pv, ncv, q->used.as<unsigned char>());
         }
         ARH_HIP(hipGetLastError());
         launched += batch;

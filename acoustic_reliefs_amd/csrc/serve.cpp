@@ -130,12 +130,15 @@ static void worker(arh_server *s, int wid) {
         arh_ctx_profile(ctx, prof_on ? 1 : 0);
         int rc = ARH_OK;
         const SMesh &m = *s->meshes[r->mesh];
+        double geom_ms = 0.0;
         if (cached != r->mesh) {
+            const auto g0 = std::chrono::steady_clock::now();
             arh_geom_destroy(geom);
             geom = nullptr;
             cached = -1;
             rc = arh_geom_create(ctx, m.P.data(), m.V, m.E.data(), m.N, m.blocks.data(), m.nd, m.na, &geom);
             if (rc == ARH_OK) cached = r->mesh;
+            geom_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
         }
         arh_solve_stats st{};
         arh_hmat_info hi{};
@@ -167,6 +170,7 @@ static void worker(arh_server *s, int wid) {
             r->info.assemble_ms = hi.assemble_ms;
             r->info.solve_ms = st.solve_ms;
             r->info.post_ms = st.post_ms;
+            r->info.geom_ms = geom_ms;
             r->info.t_done = s->now();
             if (rc != ARH_OK) r->err = arh_last_error(ctx);
             r->info.status = rc == ARH_OK ? 2 : rc;

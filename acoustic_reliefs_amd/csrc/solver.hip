@@ -15,6 +15,8 @@
 // observed convergence rate; a right-hand side that meets a restart pauses until the host
 // restarts it at the next synchronisation, which leaves its arithmetic unchanged.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -23,6 +25,31 @@
 #include "device_common.h"
 
 namespace arh {
+
+// ARH_TRACE=1 in the environment: one line per solver sweep on stderr (precision, right-hand sides
+// in the sweep, iterations, true residuals) for diagnosing refinement on hard problems.
+static bool trace_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("ARH_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
+static void trace_sweep(const char *what, int sweep, const std::vector<int> &in, const std::vector<int> &iters,
+                        const std::vector<double> &tres) {
+    if (!trace_on()) return;
+    int n = 0, imax = 0;
+    double rmax = 0.0, rmin = 1e300;
+    for (size_t j = 0; j < in.size(); j++) {
+        if (!in[j]) continue;
+        n++;
+        imax = std::max(imax, j < iters.size() ? iters[j] : 0);
+    }
+    for (double r : tres) rmax = std::max(rmax, r), rmin = std::min(rmin, r);
+    std::fprintf(stderr, "[arh solve] %s sweep %d: %d rhs, max iters %d, true residual %.3e .. %.3e\n", what, sweep, n,
+                 imax, tres.empty() ? 0.0 : rmin, rmax);
+}
 
 struct RhsState {
     double2 rho;      // rho of the previous iteration (Eigen's rho_old at the next step)
@@ -39,6 +66,8 @@ struct RhsState {
 };
 
 constexpr int kVT = 256;      // threads per block of vector kernels
+constexpr double kInnerFloor = 1e-7;  // mixed refinement: tightest inner (FP32-operator) tolerance
+constexpr int kInnerBudget = 256;      // mixed refinement: iterations an inner solve may take
 constexpr int kMaxK = 8;      // iterations per host synchronisation, at most
 constexpr int kMaxNB = 128;   // blocks per rhs of a reduction
 
@@ -323,12 +352,17 @@ struct Work {
         const int64_t vec = N * R;
         const size_t vb = sizeof(double2) * vec;
         const size_t pb = sizeof(double) * 4 * (size_t)R * NB;
-        size_t total = 7 * vb + 4 * pb + sizeof(RhsState) * R + sizeof(int) * (2 * R + 1 + kMaxK) + 9 * 256;
+        // every piece starts on a 256-byte boundary: the total is the sum of the rounded sizes
+        auto rnd = [](size_t n) { return (n + 255) / 256 * 256; };
+        const size_t total = 7 * rnd(vb) + rnd(4 * pb) + rnd(sizeof(RhsState) * R) + rnd(sizeof(int) * R) +
+                             rnd(sizeof(int) * (R + 1)) + rnd(sizeof(int) * kMaxK);
         buf.alloc(total);
         char *c = buf.as<char>();
+        char *const end = c + total;
         auto take = [&](size_t n) {
             char *q = c;
-            c += (n + 255) / 256 * 256;
+            c += rnd(n);
+            ARH_CHECK(c <= end, ARH_EINVAL, "internal: solver workspace overflow");
             return q;
         };
         r = (double2 *)take(vb), r0 = (double2 *)take(vb), p = (double2 *)take(vb), v = (double2 *)take(vb);
@@ -348,12 +382,13 @@ int pick_nb(int N) { return std::max(1, std::min(kMaxNB, (N + 2047) / 2048)); }
 // BiCGSTAB on operator `prec` (ARH_FP64 / ARH_MIXED) for the rhs with want[j] != 0, from x = 0.
 // Returns per-rhs iterations; fills recursive residual history.
 // stall_window > 0 (inner solves of the refinement only): a right-hand side whose recursive
-// residual has not halved within stall_window iterations is stopped (checked at synchronisations);
-// 0 keeps Eigen's loop control exactly.
+// residual has not halved within stall_window iterations is stopped, and so is one that has run
+// budget iterations (budget > 0) without converging (both checked at synchronisations); 0 keeps
+// Eigen's loop control exactly.
 static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 *b, double2 *x,
                      const std::vector<int> &want, const std::vector<double> &tolj, int maxit, const arh_solve_opts &o,
                      arh_solve_stats *st, std::vector<int> &iters, std::vector<double> &err, Work &W, int NB,
-                     bool record_hist, int stall_window = 0) {
+                     bool record_hist, int stall_window = 0, int budget = 0) {
     arh_ctx *ctx = h->g->ctx;
     hipStream_t s = ctx->stream;
     const int N = h->g->N;
@@ -444,15 +479,19 @@ static void bicgstab(arh_hmat *h, int transpose, int prec, int R, const double2 
             ctx->prof.slot_ok.clear();
         }
         // stalled inner solves stop here (nothing has run on the device since the state was read)
-        if (stall_window > 0) {
+        if (stall_window > 0 || budget > 0) {
             bool changed = false;
             for (int j = 0; j < R; j++) {
                 if (!(hs[j].active || hs[j].restart) || hs[j].rhs_sq == 0.0) continue;
                 const double e = std::sqrt(hs[j].rn / hs[j].rhs_sq);
-                if (e < 0.5 * best[j]) {
+                if (budget > 0 && hs[j].iters >= budget) {
+                    hs[j].active = 0;
+                    hs[j].restart = 0;
+                    changed = true;
+                } else if (e < 0.5 * best[j]) {
                     best[j] = e;
                     best_it[j] = hs[j].iters;
-                } else if (hs[j].iters - best_it[j] >= stall_window) {
+                } else if (stall_window > 0 && hs[j].iters - best_it[j] >= stall_window) {
                     hs[j].active = 0;
                     hs[j].restart = 0;
                     changed = true;
@@ -562,6 +601,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
         int sweeps = 0;
         std::vector<double> last_err = err;
         if (o.true_residual || st) true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
+        trace_sweep("fp64", 0, want, iters, tres);
         while (o.true_residual && sweeps < 8) {
             std::vector<int> redo(R, 0);
             int nre = 0;
@@ -603,22 +643,29 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
         const dim3 ga(std::max(1, std::min(1024, (N + 255) / 256)), R);
         const int stall = std::max(200, std::min(maxit, 400));
         for (int sweep = 0; sweep < 10; sweep++) {
-            // per-rhs inner tolerance relative to its current residual, aiming at half the target
-            // (FP32 operator: not below 1e-7); no coupling between right-hand sides
+            // per-rhs inner tolerance relative to its current residual, aiming at half the target (not
+            // below kInnerFloor); no coupling between right-hand sides. An inner solve gets at most
+            // kInnerBudget iterations: on a hard (ill-conditioned) problem the FP32 operator's
+            // rounding slows BiCGSTAB to two or three times the FP64 iteration count, so such a
+            // right-hand side goes to FP64 after a bounded FP32 attempt
             std::vector<double> itol(R);
             for (int j = 0; j < R; j++)
-                itol[j] = std::min(0.5, std::max(1e-7, 0.5 * tol / std::max(bnorm_rel[j], 1e-300)));
+                itol[j] = std::min(0.5, std::max(kInnerFloor, 0.5 * tol / std::max(bnorm_rel[j], 1e-300)));
             bicgstab(h, transpose, ARH_MIXED, R, rb.as<double2>(), dx.as<double2>(), redo, itol, maxit, o, st, iters,
-                     err, W, NB, sweep == 0, stall);
+                     err, W, NB, sweep == 0, stall, kInnerBudget);
             for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
             ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_axpy_cols, ga, dim3(256), 0, s, W.mask, N, x, dx.as<double2>(), 1.0);
             ARH_HIP(hipGetLastError());
             true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
             if (st) st->refinements++;
+            trace_sweep("fp32", sweep, redo, iters, tres);
             std::vector<int> undo(R, 0);
             int nundo = 0, nre = 0;
             for (int j = 0; j < R; j++) {
+                // an inner solve that stopped short of its tolerance (stalled or hit maxit) marks a
+                // problem the FP32 operator cannot resolve: finish in FP64 from here
+                if (redo[j] && tres[j] > tol && err[j] > itol[j]) fp64[j] = 1;
                 if (redo[j] && tres[j] > tol && tres[j] > 0.5 * bnorm_rel[j]) {
                     fp64[j] = 1;  // the FP32 correction did not pay: finish in FP64
                     if (tres[j] > bnorm_rel[j]) undo[j] = 1, nundo++;
@@ -653,6 +700,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             ARH_HIP(hipGetLastError());
             true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
             if (st) st->refinements++;
+            trace_sweep("fp64 correction", sweep, f64, iters, tres);
         }
         err = tres;
     }

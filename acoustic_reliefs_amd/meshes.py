@@ -94,7 +94,6 @@ def relief_box(width=0.6, depth=0.6, base=0.05, grid=64, heights=None, wall_rows
     xs = np.linspace(-width / 2, width / 2, grid)
     zs = np.linspace(-depth / 2, depth / 2, grid)
     P = []
-    E = []
 
     def add(p):
         P.append(p)
@@ -105,12 +104,13 @@ def relief_box(width=0.6, depth=0.6, base=0.05, grid=64, heights=None, wall_rows
         for ix in range(grid):
             top[iz, ix] = add((xs[ix], base + heights[iz, ix], zs[iz]))
 
+    quads, outs = [], []
+
     def quad(a, b, c, d, outward):
-        # quad a-b-c-d (in order around the boundary); choose the winding whose normal matches outward
-        for tri in ((a, b, c), (a, c, d)):
-            pa, pb, pc = (np.array(P[t]) for t in tri)
-            n = np.cross(pb - pa, pc - pa)
-            E.append(tri if np.dot(n, outward) > 0 else (tri[0], tri[2], tri[1]))
+        # quad a-b-c-d (in order around the boundary), split into (a, b, c) and (a, c, d); the winding
+        # of each triangle is chosen at the end so that its normal matches outward
+        quads.append((a, b, c, d))
+        outs.append(outward)
 
     for iz in range(grid - 1):
         for ix in range(grid - 1):
@@ -138,7 +138,16 @@ def relief_box(width=0.6, depth=0.6, base=0.05, grid=64, heights=None, wall_rows
     for iz in range(bottom_grid):
         for ix in range(bottom_grid):
             quad(bot[iz, ix], bot[iz, ix + 1], bot[iz + 1, ix + 1], bot[iz + 1, ix], np.array([0.0, -1.0, 0.0]))
-    return np.array(P, dtype=np.float64), np.array(E, dtype=np.int32), top.reshape(-1).astype(np.int32)
+    P = np.array(P, dtype=np.float64)
+    q = np.array(quads, dtype=np.int64)
+    out = np.repeat(np.array(outs), 2, axis=0)
+    tris = np.empty((2 * len(q), 3), dtype=np.int64)
+    tris[0::2] = q[:, [0, 1, 2]]
+    tris[1::2] = q[:, [0, 2, 3]]
+    n = np.cross(P[tris[:, 1]] - P[tris[:, 0]], P[tris[:, 2]] - P[tris[:, 0]])
+    flip = np.einsum("ij,ij->i", n, out) <= 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    return P, tris.astype(np.int32), top.reshape(-1).astype(np.int32)
 
 
 def relief_config(size=0.6, grid=64, hmax=0.1, seed=0, base=0.05):

@@ -215,12 +215,14 @@ def run_gradient(args):
         eng.log = log
         ctx.synchronize()
         log("engine (partition, geometry) %.2f s" % (time.perf_counter() - t_init))
+        capi.device_trim(0)  # blocks kept for reuse are not held by the engine
         free0 = torch.cuda.mem_get_info()[0]
         h = torch.zeros(len(top), dtype=torch.float64, requires_grad=True)
         t0 = time.perf_counter()
         d = autograd.diffusion_coefficients(h, eng)
         ctx.synchronize()
         t1 = time.perf_counter()
+        capi.device_trim(0)
         free1 = torch.cuda.mem_get_info()[0]
         loss = (1.0 - d).mean()
         loss.backward()
@@ -263,6 +265,191 @@ def run_gradient(args):
     print(json.dumps(out), flush=True)
 
 
+def run_diffbem(args):
+    """The drop-in module at a reference optimiser's settings (configs/base.toml: cluster_size 128, all tolerances
+    1e-5, bands 1000-3150 Hz, 1297 hemisphere listeners at 50 m with the listener ACA, point source at (0, 100, 0)):
+    DiffBEM.value() and DiffBEM.gradient(x) over every top vertex, one band alone, and the CPU oracle on that band."""
+    from acoustic_reliefs_amd import acoustics3d_impl as m
+    from acoustic_reliefs_amd import meshes
+    from oracle import oracle as O
+    P, E, top = meshes.relief_config(size=args.size, grid=args.grid, hmax=args.hmax, seed=0)
+    bands = [1000.0, 1250.0, 1600.0, 1600.0, 2000.0, 2500.0, 3150.0]
+    d = m.DiffBEM(args.leaf, 1.5, bands, 1, 1e-5, 1e-5, 1e-5, np.array([0.0, 100.0, 0.0]), 50, 5, False)
+    d.silent = True
+    Ps, Es = d.precompute(P, E, top)
+    x = np.zeros(len(top))
+    out = {"workload": "%.1f m relief, %dx%d top, N=%d, %d diff vertices, cluster_size %d, bands %s, 1297 listeners"
+                       % (args.size, args.grid, args.grid, len(E), len(top), args.leaf, bands)}
+    t0 = time.perf_counter()
+    d.value(x)
+    out["warmup_value_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    v = d.value(x)
+    out["value_s"] = time.perf_counter() - t0
+    log("value %.6f: %.2f s" % (v, out["value_s"]))
+    t0 = time.perf_counter()
+    v2, g = d.gradient(x)
+    out["gradient_s"] = time.perf_counter() - t0
+    log("gradient: %.2f s, |g| %.3e" % (out["gradient_s"], np.linalg.norm(g)))
+    f = 3150.0
+    t0 = time.perf_counter()
+    d.surface_vals(f)
+    t1 = time.perf_counter()
+    d.pvals(f)
+    t2 = time.perf_counter()
+    out["surface_vals_3150_s"], out["pvals_3150_s"] = t1 - t0, t2 - t1
+    log("3150 Hz: surface_vals %.3f s, pvals (+ listener ACA) %.3f s" % (t1 - t0, t2 - t1))
+    d.set_band(1000.0)
+    t0 = time.perf_counter()
+    vb, gb = d.gradient(x)
+    out["band_gradient_1000_s"] = time.perf_counter() - t0
+    log("1000 Hz band gradient: %.3f s" % out["band_gradient_1000_s"])
+    if not args.no_cpu:
+        threads = O.num_threads()
+        Er, dirb, appb, _ = O.setup(P, E, args.leaf, 1.5)
+        oc = O.Ctx(P, Er, dirb, appb)
+        k = 2 * np.pi * 1000.0 / 344.0
+        t0 = time.perf_counter()
+        vo, go = O.band_value_grad(oc, [k], d.get_listeners(), top, aca_tol=1e-5, solver_tol=1e-5, q_aca_tol=1e-5,
+                                   want_grad=True)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle_band_gradient_1000"] = {"s": tc, "threads": threads, "value": vo,
+                                                "abs_value_diff": abs(vo - vb),
+                                                "grad_rel_diff": float(np.linalg.norm(go - gb) / np.linalg.norm(go))}
+        log("CPU oracle 1000 Hz band gradient: %.2f s on %d threads; |d - d_gpu| %.2e, grad rel diff %.2e" % (
+            tc, threads, abs(vo - vb), out["cpu_oracle_band_gradient_1000"]["grad_rel_diff"]))
+    print(json.dumps(out), flush=True)
+
+
+def make_stream(n, pool, seed=0, rate=5.0):
+    """A seeded serving stream: `pool` distinct reliefs (size 0.3-0.9 m, top grid 16-160, heights scaled to
+    size / 6 as 0.1 m at 0.6 m and 0.15 m at 0.9 m) and n requests, each a relief drawn from the pool, a random
+    one-third-octave band and 1-37 incidence directions from the fan, arriving as a Poisson process."""
+    from acoustic_reliefs_amd import meshes
+    rng = np.random.default_rng(seed)
+    reliefs = []
+    for p in range(pool):
+        size = float(rng.uniform(0.3, 0.9))
+        grid = int(rng.integers(16, 161))
+        P, E, _ = meshes.relief_config(size=size, grid=grid, hmax=size / 6.0, seed=1000 + p)
+        reliefs.append({"size": size, "grid": grid, "P": P, "E": E})
+    fan = meshes.incidence_directions(37)
+    t, reqs = 0.0, []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        nr = int(rng.integers(1, 38))
+        reqs.append({"t": t, "relief": int(rng.integers(pool)), "band": float(rng.choice(meshes.THIRD_OCTAVE_BANDS)),
+                     "dirs": fan[np.sort(rng.choice(37, nr, replace=False))]})
+    return reliefs, reqs
+
+
+def run_stream(args):
+    """Serving stream: requests submitted to the continuous-batching server at their Poisson arrival times
+    (mixed precision, true residual 1e-6); latency = completion - arrival. Baseline: the first
+    --baseline requests one at a time on the complex-FP64 path (each request's directions in one solve)."""
+    import threading
+    from acoustic_reliefs_amd import capi, meshes
+    reliefs, reqs = make_stream(args.requests, args.pool, seed=0, rate=args.rate)
+    L = meshes.semicircle_receivers(5.0, 181)
+    srv = capi.Server(workers=args.workers)
+    t0 = time.perf_counter()
+    mids = [srv.add_mesh(r["P"], r["E"], leaf=32, factor=1.5) for r in reliefs]
+    log("stream: %d requests over %d reliefs (N %d-%d), rate %.2f/s; partitions %.1f s" % (
+        len(reqs), len(reliefs), min(len(r["E"]) for r in reliefs), max(len(r["E"]) for r in reliefs), args.rate,
+        time.perf_counter() - t0))
+    rids = [None] * len(reqs)
+    submitted = threading.Event()
+
+    def client():
+        start = time.perf_counter()
+        for i, q in enumerate(reqs):
+            dt = q["t"] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            rids[i] = srv.submit(mids[q["relief"]], 2 * np.pi * q["band"] / C0, q["dirs"], L,
+                                 kind=capi.ARH_RHS_PLANE, storage=capi.ARH_MIXED, aca_tol=1e-4, tol=1e-6)
+        submitted.set()
+
+    th = threading.Thread(target=client)
+    th.start()
+    res = []
+    try:
+        for i in range(len(reqs)):
+            while rids[i] is None:
+                time.sleep(0.001)
+            d, info = srv.wait(rids[i])
+            res.append((d, info))
+            if (i + 1) % 25 == 0 or i + 1 == len(reqs):
+                log("  %d / %d done (request %d: %d rhs, N %d, %g Hz, service %.3f s, iters %d)" % (
+                    i + 1, len(reqs), i, len(reqs[i]["dirs"]), len(reliefs[reqs[i]["relief"]]["E"]),
+                    reqs[i]["band"], info["t_done"] - info["t_start"], info["iters"]))
+    finally:
+        # the server's workers stop (after their current request) before the interpreter goes
+        th.join()
+        srv.close()
+    t_first = min(i["t_submit"] for _, i in res)
+    t_last = max(i["t_done"] for _, i in res)
+    solves = sum(len(q["dirs"]) for q in reqs)
+    lat = np.array([i["t_done"] - i["t_submit"] for _, i in res])
+    worst = max(i["max_true_res"] for _, i in res)
+    out = {"workload": "serving stream: %d Poisson requests (rate %.2f/s) over %d seeded reliefs 0.3-0.9 m, "
+                       "top grids 16-160 (N %d-%d), random band 250-4000 Hz, 1-37 plane waves, 181 receivers"
+                       % (len(reqs), args.rate, len(reliefs), min(len(r["E"]) for r in reliefs),
+                          max(len(r["E"]) for r in reliefs), ),
+           "solves": solves, "span_s": t_last - t_first, "solves_per_s": solves / (t_last - t_first),
+           "requests_per_s": len(reqs) / (t_last - t_first), "p50_latency_s": float(np.percentile(lat, 50)),
+           "p99_latency_s": float(np.percentile(lat, 99)), "max_true_residual": worst,
+           "workers": args.workers}
+    log("stream: %d solves in %.2f s = %.1f solves/s; latency p50 %.3f s, p99 %.3f s; max true residual %.2e" % (
+        solves, t_last - t_first, out["solves_per_s"], out["p50_latency_s"], out["p99_latency_s"], worst))
+    recs = [{"n": len(reliefs[q["relief"]]["E"]), "nrhs": len(q["dirs"]), "band": q["band"],
+             "queue_s": i["t_start"] - i["t_submit"], "service_s": i["t_done"] - i["t_start"],
+             "geom_ms": i["geom_ms"], "assemble_ms": i["assemble_ms"], "solve_ms": i["solve_ms"],
+             "post_ms": i["post_ms"], "iters": i["iters"]} for q, (_, i) in zip(reqs, res)]
+    if args.baseline:
+        # the same requests one at a time on the complex-FP64 path, with the same per-request work as the
+        # server: a geometry built from the precomputed partition whenever the relief changes, one band
+        # solve of the request's directions
+        ctx = capi.Context(0)
+        parts = {}
+        cur, g = None, None
+        t_fp64, n_fp64, dmax = 0.0, 0, 0.0
+        for i, q in enumerate(reqs[:args.baseline]):
+            r = q["relief"]
+            if r not in parts:
+                P, E = reliefs[r]["P"], reliefs[r]["E"]
+                perm, dense, lowrank = capi.partition(capi.centroids(P, E), 32, 1.5)
+                parts[r] = (P, np.asarray(E, np.int32)[perm], dense, lowrank)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            if cur != r:
+                g = None
+                g = capi.Geometry(ctx, *parts[r])
+                cur = r
+            d64, _, _, st, _ = g.band_solve(2 * np.pi * q["band"] / C0, capi.ARH_RHS_PLANE, q["dirs"], L,
+                                            aca_tol=1e-4, storage=capi.ARH_FP64, tol=1e-6, true_residual=True)
+            dt = time.perf_counter() - t1
+            t_fp64 += dt
+            n_fp64 += len(q["dirs"])
+            recs[i]["fp64_s"] = dt
+            dmax = max(dmax, float(np.abs(d64 - res[i][0]).max()))
+            if (i + 1) % 25 == 0:
+                log("  fp64 baseline %d / %d" % (i + 1, args.baseline))
+        g = None
+        out["fp64_one_request_at_a_time"] = {"requests": min(args.baseline, len(reqs)), "solves": n_fp64,
+                                             "busy_s": t_fp64, "solves_per_s": n_fp64 / t_fp64,
+                                             "max_abs_d_diff_vs_served": dmax}
+        nb = min(args.baseline, len(reqs))
+        if nb == len(reqs):
+            out["served_over_fp64"] = out["solves_per_s"] / (n_fp64 / t_fp64)
+        log("fp64 one request at a time (first %d requests): %.1f solves/s; max |d_mixed - d_fp64| %.2e" % (
+            nb, n_fp64 / t_fp64, dmax))
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(recs, f)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +467,15 @@ def main():
     ap.add_argument("--fd", type=int, default=0, help="with --gradient: central differences on this many vertices")
     ap.add_argument("--size", type=float, default=0.6, help="with --gradient: relief width (m)")
     ap.add_argument("--hmax", type=float, default=0.1, help="with --gradient: maximum height (m)")
+    ap.add_argument("--stream", action="store_true", help="the serving stream (Poisson requests over random reliefs)")
+    ap.add_argument("--requests", type=int, default=2000, help="with --stream: number of requests")
+    ap.add_argument("--pool", type=int, default=64, help="with --stream: distinct reliefs")
+    ap.add_argument("--rate", type=float, default=5.0, help="with --stream: arrivals per second")
+    ap.add_argument("--baseline", type=int, default=40, help="with --stream: requests in the FP64 baseline")
+    ap.add_argument("--out", type=str, default="", help="with --stream: write per-request records (JSON) here")
+    ap.add_argument("--diffbem", action="store_true",
+                    help="the drop-in DiffBEM at a reference optimiser's settings (value, gradient, CPU oracle band)")
+    ap.add_argument("--leaf", type=int, default=128, help="with --diffbem: cluster_size")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated timing of the dominant kernel (the roofline figures)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -290,6 +486,12 @@ def main():
         return
     if args.gradient:
         run_gradient(args)
+        return
+    if args.diffbem:
+        run_diffbem(args)
+        return
+    if args.stream:
+        run_stream(args)
         return
     if args.roofline_only:
         from acoustic_reliefs_amd import capi

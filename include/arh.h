@@ -121,6 +121,13 @@ int arh_ctx_profile(arh_ctx *ctx, int enable);
 int arh_ctx_get_profile(arh_ctx *ctx, arh_profile *out);
 
 /* Device memory helpers for FFI callers without a HIP binding. */
+/* New (the reference has no device memory). The library keeps the device blocks its handles free
+ * and hands them out again (a band or request loop then stops mapping fresh memory); a block is
+ * reused on another stream only after the work of its previous stream. arh_device_trim returns
+ * every cached block of a device to HIP; arh_device_cached reports the bytes held. Setting
+ * ARH_POOL=0 in the environment disables the cache. */
+int arh_device_trim(int device, size_t *freed_bytes);
+int arh_device_cached(int device, size_t *bytes);
 int arh_malloc(arh_ctx *ctx, size_t bytes, void **dptr);
 int arh_free(arh_ctx *ctx, void *dptr);
 int arh_upload(arh_ctx *ctx, void *dst_device, const void *src_host, size_t bytes);
@@ -255,6 +262,7 @@ typedef struct {
     int iters;             /* BiCGSTAB iterations (max over rhs) */
     double max_true_res;   /* FP64 true residual (max over rhs) */
     double assemble_ms, solve_ms, post_ms; /* split of the service time (wall, on the worker) */
+    double geom_ms;        /* device geometry set-up when the worker's cached mesh changes (wall) */
 } arh_request_info;
 
 int arh_server_create(const arh_server_opts *opts, arh_server **out);

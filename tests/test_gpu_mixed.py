@@ -99,3 +99,22 @@ def test_compact_keeps_results(relief, capi):
     after = [h.matvec_np(X, transpose=t, precision=p) for t in (False, True) for p in (capi.ARH_MIXED, capi.ARH_FP64)]
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("grid", [16, 17])
+def test_solver_workspace_every_batch_size(ctx, capi, grid):
+    """Regression: the solver's workspace pieces are 256-byte aligned, so its size is the sum of the rounded
+    pieces; with a fixed slack some (N, nrhs) put the per-batch active counts past the allocation
+    (N = 818, one rhs, failed with an invalid copy). Every batch size 1..40 on two small reliefs, both
+    precisions, must solve to the true-residual tolerance."""
+    from acoustic_reliefs_amd import meshes
+    P, E, _ = meshes.relief_config(0.3, grid, 0.05, seed=1000 + grid)
+    g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
+    k = 2 * np.pi * 1250.0 / 343.0
+    L = meshes.semicircle_receivers(5.0, 181)
+    fan = meshes.incidence_directions(40)
+    print("workspace sizes: N=%d" % g.N, flush=True)
+    for storage in (capi.ARH_FP64, capi.ARH_MIXED):
+        for R in range(1, 41):
+            d, _, _, st, _ = g.band_solve(k, capi.ARH_RHS_PLANE, fan[:R], L, storage=storage, tol=1e-6)
+            assert st["max_true_res"] <= 1e-6 and np.all(np.isfinite(d)), (storage, R)
