@@ -63,7 +63,7 @@ class Profile(ctypes.Structure):
 
 
 class ServerOpts(ctypes.Structure):
-    _fields_ = [("workers", c_int), ("hbm_budget", ctypes.c_int64), ("device", c_int)]
+    _fields_ = [("workers", c_int), ("hbm_budget", ctypes.c_int64), ("device", c_int), ("maxit", c_int)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -365,14 +365,14 @@ class Geometry:
         return A.numpy()[:HH], B.numpy()[:HH], C.numpy()[:HH]
 
     def band_solve(self, k, kind, vecs, L, aca_tol=1e-4, max_rank=100, storage=ARH_FP64, tol=1e-6,
-                   true_residual=True, want_x=False, want_y=False):
+                   true_residual=True, want_x=False, want_y=False, maxit=0):
         vecs = np.ascontiguousarray(np.asarray(vecs, dtype=np.float64).reshape(-1, 3))
         L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 3)
         R, LL = len(vecs), len(L)
         d = np.zeros(R)
         x = self.ctx.empty((R, self.N)) if want_x else None
         y = self.ctx.empty((R, LL)) if want_y else None
-        opts = SolveOpts(tol=tol, maxit=0, true_residual=int(true_residual), fail_on_noconv=0, hist_cap=0)
+        opts = SolveOpts(tol=tol, maxit=int(maxit), true_residual=int(true_residual), fail_on_noconv=0, hist_cap=0)
         it = np.zeros(R, np.int32)
         tr = np.zeros(R)
         st = SolveStats(iters_per_rhs=_ip(it), true_res=_dp(tr))
@@ -510,9 +510,9 @@ class HMatrix:
 class Server:
     """Continuously batched band solves (arh_server_*): submit returns at once, wait blocks."""
 
-    def __init__(self, workers=4, hbm_budget=0, device=0):
+    def __init__(self, workers=4, hbm_budget=0, device=0, maxit=0):
         h = c_void_p()
-        opts = ServerOpts(workers=int(workers), hbm_budget=int(hbm_budget), device=int(device))
+        opts = ServerOpts(workers=int(workers), hbm_budget=int(hbm_budget), device=int(device), maxit=int(maxit))
         _check(lib().arh_server_create(ctypes.byref(opts), ctypes.byref(h)))
         self.h = h
         self._nrhs = {}

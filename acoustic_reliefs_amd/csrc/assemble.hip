@@ -377,6 +377,12 @@ void assemble_aca(arh_hmat *h) {
         ARH_HIP(hipGetLastError());
     }
     ARH_HIP(hipStreamSynchronize(s));
+    // the scratch goes back to the device memory cache (the next assembly takes it from there), so an
+    // idle geometry holds no ACA scratch
+    g->aca_U.reset();
+    g->aca_V.reset();
+    g->aca_flags.reset();
+    g->scratch_rank = -1;
 }
 
 // FP32 copy for the generic (non-MFMA) product: dense panels and U elementwise, V column-major

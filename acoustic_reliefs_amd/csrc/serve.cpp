@@ -39,6 +39,7 @@ struct Request {
     std::vector<double> vecs, L;
     std::vector<double> d;
     int64_t footprint = 0;
+    bool exclusive = false;  // out of memory once: runs again alone, after a trim of the memory cache
     arh_request_info info{};
     std::string err;
 };
@@ -117,7 +118,7 @@ static void worker(arh_server *s, int wid) {
                 if (s->stop) return true;
                 if (s->queue.empty()) return false;
                 const Request &h = *s->reqs[s->queue.front()];
-                return s->used == 0 || s->used + h.footprint <= s->budget;
+                return s->used == 0 || (!h.exclusive && s->used + h.footprint <= s->budget);
             });
             if (s->stop) break;
             r = s->reqs[s->queue.front()].get();
@@ -146,9 +147,24 @@ static void worker(arh_server *s, int wid) {
         if (rc == ARH_OK) {
             arh_solve_opts o{};
             o.tol = r->tol;
+            o.maxit = s->opts.maxit;
             o.true_residual = 1;
             rc = arh_band_solve(geom, r->k, r->aca_tol, 100, r->storage, r->kind, r->nrhs, r->vecs.data(),
                                 r->L.data(), (int)r->L.size() / 3, &o, d.data(), nullptr, nullptr, &st, &hi);
+        }
+        if (rc == ARH_ENOMEM && !r->exclusive) {
+            // the footprint estimate let in more than the device holds: free what the memory cache
+            // keeps and run the request again when nothing else is in flight
+            arh_device_trim(s->opts.device, nullptr);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->used -= r->footprint;
+                r->exclusive = true;
+                r->info.status = 0;
+                s->queue.push_front(r->id);
+            }
+            s->cv_work.notify_all();
+            continue;
         }
         arh_profile pr{};
         if (prof_on) arh_ctx_get_profile(ctx, &pr);
@@ -188,6 +204,7 @@ int arh_server_create(const arh_server_opts *opts, arh_server **out) {
     return sguard([&] {
         ARH_CHECK(opts && out, ARH_EINVAL, "NULL argument");
         ARH_CHECK(opts->workers >= 1 && opts->workers <= 16, ARH_EINVAL, "workers must be in [1, 16]");
+        ARH_CHECK(opts->maxit >= 0, ARH_EINVAL, "maxit must be >= 0");
         auto s = std::make_unique<arh_server>();
         s->opts = *opts;
         ARH_HIP(hipSetDevice(opts->device));

@@ -597,17 +597,20 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
     if (h->storage != ARH_MIXED) {
         bicgstab(h, transpose, ARH_FP64, R, b, x, want, std::vector<double>(R, tol), maxit, o, st, iters, err, W, NB, true);
         for (int j = 0; j < R; j++) iters_tot[j] = iters[j];
-        // optional: continue from x until the FP64 true residual meets tol
+        // optional: continue from x until the FP64 true residual meets tol; the correction solves
+        // share what is left of maxit and stop when one no longer halves the worst residual
         int sweeps = 0;
+        int left = maxit - *std::max_element(iters.begin(), iters.end());
         std::vector<double> last_err = err;
         if (o.true_residual || st) true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
         trace_sweep("fp64", 0, want, iters, tres);
-        while (o.true_residual && sweeps < 8) {
+        while (o.true_residual && sweeps < 8 && left > 0) {
             std::vector<int> redo(R, 0);
             int nre = 0;
             for (int j = 0; j < R; j++)
                 if (tres[j] > tol) redo[j] = 1, nre++;
             if (!nre) break;
+            const double worst = *std::max_element(tres.begin(), tres.end());
             // correction solve A d = b - A x for the unconverged rhs, then x += d
             DBuf rb, dx;
             rb.alloc(sizeof(double2) * (size_t)N * R);
@@ -616,8 +619,9 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             // per-rhs tolerance relative to the current residual (no coupling between right-hand sides)
             std::vector<double> ctol(R);
             for (int j = 0; j < R; j++) ctol[j] = std::min(0.5, 0.5 * tol / std::max(tres[j], 1e-300));
-            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), redo, ctol, maxit, o, st, iters,
+            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), redo, ctol, left, o, st, iters,
                      err, W, NB, false);
+            left -= *std::max_element(iters.begin(), iters.end());
             ARH_HIP(hipMemcpyAsync(W.mask, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_axpy_cols, dim3(std::max(1, std::min(1024, (N + 255) / 256)), R), dim3(256), 0, s,
                                W.mask, N, x, dx.as<double2>(), 1.0);
@@ -626,6 +630,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             true_residual(h, transpose, R, b, x, nullptr, W, NB, tres, st);
             sweeps++;
             if (st) st->refinements++;
+            if (*std::max_element(tres.begin(), tres.end()) > 0.5 * worst) break;
         }
         err = last_err;
     } else {
@@ -683,24 +688,32 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             }
             if (!nre) break;
         }
-        // FP64 correction solves for the right-hand sides still above tol
-        for (int sweep = 0; sweep < 8; sweep++) {
+        // FP64 correction solves for the right-hand sides still above tol; together they get maxit
+        // iterations and stop when one no longer halves the worst residual
+        int left = maxit;
+        for (int sweep = 0; sweep < 8 && left > 0; sweep++) {
             std::vector<int> f64(R, 0);
             int nf = 0;
+            double worst = 0.0;
             for (int j = 0; j < R; j++)
-                if (tres[j] > tol) f64[j] = 1, nf++;
+                if (tres[j] > tol) f64[j] = 1, nf++, worst = std::max(worst, tres[j]);
             if (!nf) break;
             std::vector<double> ctol(R);
             for (int j = 0; j < R; j++) ctol[j] = std::min(0.5, 0.5 * tol / std::max(tres[j], 1e-300));
-            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), f64, ctol, maxit, o, st, iters,
+            bicgstab(h, transpose, ARH_FP64, R, rb.as<double2>(), dx.as<double2>(), f64, ctol, left, o, st, iters,
                      err, W, NB, false);
             for (int j = 0; j < R; j++) iters_tot[j] += f64[j] ? iters[j] : 0;
+            left -= *std::max_element(iters.begin(), iters.end());
             ARH_HIP(hipMemcpyAsync(W.mask, f64.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_axpy_cols, ga, dim3(256), 0, s, W.mask, N, x, dx.as<double2>(), 1.0);
             ARH_HIP(hipGetLastError());
             true_residual(h, transpose, R, b, x, rb.as<double2>(), W, NB, tres, st);
             if (st) st->refinements++;
             trace_sweep("fp64 correction", sweep, f64, iters, tres);
+            double now = 0.0;
+            for (int j = 0; j < R; j++)
+                if (f64[j]) now = std::max(now, tres[j]);
+            if (now > 0.5 * worst) break;
         }
         err = tres;
     }

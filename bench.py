@@ -343,78 +343,111 @@ def make_stream(n, pool, seed=0, rate=5.0):
     return reliefs, reqs
 
 
+def fifo_latencies(arrivals, service):
+    """Completion - arrival of each request for one server taking requests in arrival order."""
+    t, out = 0.0, []
+    for a, sv in zip(arrivals, service):
+        t = max(t, a) + sv
+        out.append(t - a)
+    return np.array(out)
+
+
 def run_stream(args):
-    """Serving stream: requests submitted to the continuous-batching server at their Poisson arrival times
-    (mixed precision, true residual 1e-6); latency = completion - arrival. Baseline: the first
-    --baseline requests one at a time on the complex-FP64 path (each request's directions in one solve)."""
+    """Serving stream (--phase served): the requests are submitted to the continuous-batching server at
+    their Poisson arrival times (mixed precision, true residual 1e-6); latency = completion - arrival.
+    --phase fp64: the same requests one at a time on the complex-FP64 path (each request's directions in
+    one batched solve, a geometry set up whenever the relief changes, as the server does); its latency is
+    that of a one-at-a-time server fed the same arrivals. --phase both runs the two. Both paths stop a
+    request's BiCGSTAB at --maxit iterations (near the closed box's interior resonances a few requests
+    do not reach 1e-6 within 2N); such requests are counted, not retried."""
     import threading
     from acoustic_reliefs_amd import capi, meshes
     reliefs, reqs = make_stream(args.requests, args.pool, seed=0, rate=args.rate)
     L = meshes.semicircle_receivers(5.0, 181)
-    srv = capi.Server(workers=args.workers)
-    t0 = time.perf_counter()
-    mids = [srv.add_mesh(r["P"], r["E"], leaf=32, factor=1.5) for r in reliefs]
-    log("stream: %d requests over %d reliefs (N %d-%d), rate %.2f/s; partitions %.1f s" % (
-        len(reqs), len(reliefs), min(len(r["E"]) for r in reliefs), max(len(r["E"]) for r in reliefs), args.rate,
-        time.perf_counter() - t0))
-    rids = [None] * len(reqs)
-    submitted = threading.Event()
-
-    def client():
-        start = time.perf_counter()
-        for i, q in enumerate(reqs):
-            dt = q["t"] - (time.perf_counter() - start)
-            if dt > 0:
-                time.sleep(dt)
-            rids[i] = srv.submit(mids[q["relief"]], 2 * np.pi * q["band"] / C0, q["dirs"], L,
-                                 kind=capi.ARH_RHS_PLANE, storage=capi.ARH_MIXED, aca_tol=1e-4, tol=1e-6)
-        submitted.set()
-
-    th = threading.Thread(target=client)
-    th.start()
-    res = []
-    try:
-        for i in range(len(reqs)):
-            while rids[i] is None:
-                time.sleep(0.001)
-            d, info = srv.wait(rids[i])
-            res.append((d, info))
-            if (i + 1) % 25 == 0 or i + 1 == len(reqs):
-                log("  %d / %d done (request %d: %d rhs, N %d, %g Hz, service %.3f s, iters %d)" % (
-                    i + 1, len(reqs), i, len(reqs[i]["dirs"]), len(reliefs[reqs[i]["relief"]]["E"]),
-                    reqs[i]["band"], info["t_done"] - info["t_start"], info["iters"]))
-    finally:
-        # the server's workers stop (after their current request) before the interpreter goes
-        th.join()
-        srv.close()
-    t_first = min(i["t_submit"] for _, i in res)
-    t_last = max(i["t_done"] for _, i in res)
+    nmin, nmax = min(len(r["E"]) for r in reliefs), max(len(r["E"]) for r in reliefs)
     solves = sum(len(q["dirs"]) for q in reqs)
-    lat = np.array([i["t_done"] - i["t_submit"] for _, i in res])
-    worst = max(i["max_true_res"] for _, i in res)
     out = {"workload": "serving stream: %d Poisson requests (rate %.2f/s) over %d seeded reliefs 0.3-0.9 m, "
-                       "top grids 16-160 (N %d-%d), random band 250-4000 Hz, 1-37 plane waves, 181 receivers"
-                       % (len(reqs), args.rate, len(reliefs), min(len(r["E"]) for r in reliefs),
-                          max(len(r["E"]) for r in reliefs), ),
-           "solves": solves, "span_s": t_last - t_first, "solves_per_s": solves / (t_last - t_first),
-           "requests_per_s": len(reqs) / (t_last - t_first), "p50_latency_s": float(np.percentile(lat, 50)),
-           "p99_latency_s": float(np.percentile(lat, 99)), "max_true_residual": worst,
-           "workers": args.workers}
-    log("stream: %d solves in %.2f s = %.1f solves/s; latency p50 %.3f s, p99 %.3f s; max true residual %.2e" % (
-        solves, t_last - t_first, out["solves_per_s"], out["p50_latency_s"], out["p99_latency_s"], worst))
-    recs = [{"n": len(reliefs[q["relief"]]["E"]), "nrhs": len(q["dirs"]), "band": q["band"],
-             "queue_s": i["t_start"] - i["t_submit"], "service_s": i["t_done"] - i["t_start"],
-             "geom_ms": i["geom_ms"], "assemble_ms": i["assemble_ms"], "solve_ms": i["solve_ms"],
-             "post_ms": i["post_ms"], "iters": i["iters"]} for q, (_, i) in zip(reqs, res)]
-    if args.baseline:
-        # the same requests one at a time on the complex-FP64 path, with the same per-request work as the
-        # server: a geometry built from the precomputed partition whenever the relief changes, one band
-        # solve of the request's directions
+                       "top grids 16-160 (N %d-%d), random band 250-4000 Hz, 1-37 plane waves (%d solves), "
+                       "181 receivers" % (len(reqs), args.rate, len(reliefs), nmin, nmax, solves)}
+    recs = [{"t": q["t"], "n": len(reliefs[q["relief"]]["E"]), "nrhs": len(q["dirs"]), "band": q["band"]}
+            for q in reqs]
+    d_served = None
+    if args.phase in ("served", "both"):
+        srv = capi.Server(workers=args.workers, maxit=args.maxit)
+        t0 = time.perf_counter()
+        mids = [srv.add_mesh(r["P"], r["E"], leaf=32, factor=1.5) for r in reliefs]
+        log("stream: %d requests over %d reliefs (N %d-%d), rate %.2f/s; partitions %.1f s" % (
+            len(reqs), len(reliefs), nmin, nmax, args.rate, time.perf_counter() - t0))
+        rids = [None] * len(reqs)
+        stop = threading.Event()
+
+        def client():
+            start = time.perf_counter()
+            for i, q in enumerate(reqs):
+                dt = q["t"] - (time.perf_counter() - start)
+                if dt > 0 and stop.wait(dt):
+                    return
+                rids[i] = srv.submit(mids[q["relief"]], 2 * np.pi * q["band"] / C0, q["dirs"], L,
+                                     kind=capi.ARH_RHS_PLANE, storage=capi.ARH_MIXED, aca_tol=1e-4, tol=1e-6)
+
+        th = threading.Thread(target=client)
+        th.start()
+        res = []
+        try:
+            last = time.perf_counter()
+            for i in range(len(reqs)):
+                while rids[i] is None:
+                    time.sleep(0.001)
+                # wait in order, printing a line at least every 30 s while a long request runs
+                while srv.poll(rids[i])["status"] in (0, 1):
+                    time.sleep(0.01)
+                    if time.perf_counter() - last > 30.0:
+                        log("  waiting for request %d (N %d, %g Hz, %d rhs)" % (
+                            i, recs[i]["n"], recs[i]["band"], recs[i]["nrhs"]))
+                        last = time.perf_counter()
+                try:
+                    d, info = srv.wait(rids[i])
+                except RuntimeError as e:
+                    # a failed request is reported and counted; the stream goes on
+                    log("  request %d (N %d, %g Hz, %d rhs) failed: %s" % (
+                        i, recs[i]["n"], recs[i]["band"], recs[i]["nrhs"], e))
+                    info = srv.poll(rids[i])
+                    info["max_true_res"] = float("inf")
+                    d = np.full(recs[i]["nrhs"], np.nan)
+                    recs[i]["error"] = str(e)
+                res.append((d, info))
+                if (i + 1) % 50 == 0 or i + 1 == len(reqs):
+                    log("  %d / %d done (latency %.3f s)" % (i + 1, len(reqs), info["t_done"] - info["t_submit"]))
+                    last = time.perf_counter()
+        finally:
+            # the server's workers stop (after their current request) before the interpreter goes
+            stop.set()
+            th.join()
+            srv.close()
+        t_first = min(i["t_submit"] for _, i in res)
+        t_last = max(i["t_done"] for _, i in res)
+        lat = np.array([i["t_done"] - i["t_submit"] for _, i in res])
+        out["served"] = {"workers": args.workers, "maxit": args.maxit,
+                         "failed_requests": int(sum("error" in r for r in recs)),
+                         "unconverged_requests": int(sum(i["max_true_res"] > 1e-6 for _, i in res)),
+                         "span_s": t_last - t_first,
+                         "solves_per_s": solves / (t_last - t_first), "requests_per_s": len(reqs) / (t_last - t_first),
+                         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                         "max_true_residual": max(i["max_true_res"] for _, i in res)}
+        for r, (_, i) in zip(recs, res):
+            r.update({"latency_s": i["t_done"] - i["t_submit"], "service_s": i["t_done"] - i["t_start"],
+                      "geom_ms": i["geom_ms"], "assemble_ms": i["assemble_ms"], "solve_ms": i["solve_ms"],
+                      "post_ms": i["post_ms"], "iters": i["iters"]})
+        d_served = [d for d, _ in res]
+        log("served: %d solves in %.2f s = %.1f solves/s; latency p50 %.3f s, p99 %.3f s" % (
+            solves, t_last - t_first, out["served"]["solves_per_s"], out["served"]["p50_latency_s"],
+            out["served"]["p99_latency_s"]))
+    if args.phase in ("fp64", "both"):
         ctx = capi.Context(0)
         parts = {}
         cur, g = None, None
-        t_fp64, n_fp64, dmax = 0.0, 0, 0.0
-        for i, q in enumerate(reqs[:args.baseline]):
+        svc, dmax, nconv = [], 0.0, 0
+        for i, q in enumerate(reqs):
             r = q["relief"]
             if r not in parts:
                 P, E = reliefs[r]["P"], reliefs[r]["E"]
@@ -427,23 +460,28 @@ def run_stream(args):
                 g = capi.Geometry(ctx, *parts[r])
                 cur = r
             d64, _, _, st, _ = g.band_solve(2 * np.pi * q["band"] / C0, capi.ARH_RHS_PLANE, q["dirs"], L,
-                                            aca_tol=1e-4, storage=capi.ARH_FP64, tol=1e-6, true_residual=True)
-            dt = time.perf_counter() - t1
-            t_fp64 += dt
-            n_fp64 += len(q["dirs"])
-            recs[i]["fp64_s"] = dt
-            dmax = max(dmax, float(np.abs(d64 - res[i][0]).max()))
-            if (i + 1) % 25 == 0:
-                log("  fp64 baseline %d / %d" % (i + 1, args.baseline))
+                                            aca_tol=1e-4, storage=capi.ARH_FP64, tol=1e-6, true_residual=True,
+                                            maxit=args.maxit)
+            svc.append(time.perf_counter() - t1)
+            nconv += st["max_true_res"] > 1e-6
+            recs[i]["fp64_s"] = svc[-1]
+            if d_served is not None:
+                dmax = max(dmax, float(np.abs(d64 - d_served[i]).max()))
+            if (i + 1) % 50 == 0:
+                log("  fp64 %d / %d" % (i + 1, len(reqs)))
         g = None
-        out["fp64_one_request_at_a_time"] = {"requests": min(args.baseline, len(reqs)), "solves": n_fp64,
-                                             "busy_s": t_fp64, "solves_per_s": n_fp64 / t_fp64,
-                                             "max_abs_d_diff_vs_served": dmax}
-        nb = min(args.baseline, len(reqs))
-        if nb == len(reqs):
-            out["served_over_fp64"] = out["solves_per_s"] / (n_fp64 / t_fp64)
-        log("fp64 one request at a time (first %d requests): %.1f solves/s; max |d_mixed - d_fp64| %.2e" % (
-            nb, n_fp64 / t_fp64, dmax))
+        lat = fifo_latencies([q["t"] for q in reqs], svc)
+        busy = float(np.sum(svc))
+        out["fp64_one_at_a_time"] = {"busy_s": busy, "solves_per_s_capacity": solves / busy,
+                                     "unconverged_requests": int(nconv),
+                                     "p50_latency_s": float(np.percentile(lat, 50)),
+                                     "p99_latency_s": float(np.percentile(lat, 99)),
+                                     "latency_note": "one server taking the same arrivals in order"}
+        if d_served is not None:
+            out["fp64_one_at_a_time"]["max_abs_d_diff_vs_served"] = dmax
+        log("fp64 one at a time: busy %.1f s = %.1f solves/s capacity; latency at the same arrivals p50 %.3f s, "
+            "p99 %.3f s" % (busy, solves / busy, out["fp64_one_at_a_time"]["p50_latency_s"],
+                            out["fp64_one_at_a_time"]["p99_latency_s"]))
     if args.out:
         with open(args.out, "w") as f:
             json.dump(recs, f)
@@ -471,7 +509,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="with --stream: number of requests")
     ap.add_argument("--pool", type=int, default=64, help="with --stream: distinct reliefs")
     ap.add_argument("--rate", type=float, default=5.0, help="with --stream: arrivals per second")
-    ap.add_argument("--baseline", type=int, default=40, help="with --stream: requests in the FP64 baseline")
+    ap.add_argument("--maxit", type=int, default=10000, help="with --stream: BiCGSTAB iteration limit per request")
+    ap.add_argument("--phase", choices=["served", "fp64", "both"], default="both",
+                    help="with --stream: the served stream, the one-at-a-time FP64 pass, or both")
     ap.add_argument("--out", type=str, default="", help="with --stream: write per-request records (JSON) here")
     ap.add_argument("--diffbem", action="store_true",
                     help="the drop-in DiffBEM at a reference optimiser's settings (value, gradient, CPU oracle band)")

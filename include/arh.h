@@ -252,6 +252,7 @@ typedef struct {
     int workers;        /* concurrent requests (device streams) */
     int64_t hbm_budget; /* bytes the in-flight requests may hold (0 = 90% of free HBM at creation) */
     int device;
+    int maxit;          /* BiCGSTAB iteration limit per request (0 = 2N, Eigen's default) */
 } arh_server_opts;
 
 typedef struct {
