@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["geometry.hip", "assemble.hip", "matvec.hip", "matvec64.hip", "matvec_mfma.hip", "solver.hip", "receivers.hip",
-               "grad.hip", "qaca.hip", "capi.cpp", "partition.cpp", "serve.cpp", "pool.cpp"]
+               "grad.hip", "qaca.hip", "relief.hip", "capi.cpp", "partition.cpp", "serve.cpp", "pool.cpp"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
 

@@ -107,6 +107,10 @@ SIGNATURES = [
                           ctypes.POINTER(SolveStats)]),
     ("arh_rhs", c_int, [c_void_p, c_double, c_int, c_int, P_D, c_void_p]),
     ("arh_receivers", c_int, [c_void_p, c_double, P_D, c_int, c_int, c_int, c_void_p, c_void_p]),
+    ("arh_relief_create", c_int, [c_void_p, P_D, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    ("arh_relief_eval", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("arh_relief_adjoint", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("arh_relief_destroy", None, [c_void_p]),
     ("arh_qaca_create", c_int, [c_void_p, c_double, P_D, c_int, c_double, c_int, ctypes.POINTER(c_void_p)]),
     ("arh_qaca_rank", c_int, [c_void_p, P_I]),
     ("arh_qaca_apply", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
@@ -211,6 +215,7 @@ class Context:
         h = c_void_p()
         _check(lib().arh_ctx_create(int(device), ctypes.byref(h)))
         self.h = h
+        self.device = int(device)
 
     def close(self):
         if getattr(self, "h", None) and not _EXITING:
@@ -384,6 +389,31 @@ class Geometry:
                  "max_err": st.max_err, "max_true_res": st.max_true_res, "iters_per_rhs": it, "true_res": tr,
                  "solve_ms": st.solve_ms, "post_ms": st.post_ms}
         return d, x, y, stats, info.as_dict()
+
+
+class Relief:
+    """Heightfield -> differentiable-vertex heights and the adjoint (arh_relief_*), for fixed uvs."""
+
+    def __init__(self, ctx, uvs, tex_h, tex_w):
+        self.ctx = ctx
+        uvs = np.ascontiguousarray(uvs, dtype=np.float64).reshape(-1, 2)
+        self.n, self.shape = len(uvs), (int(tex_h), int(tex_w))
+        h = c_void_p()
+        _check(lib().arh_relief_create(ctx.h, _dp(uvs), self.n, int(tex_h), int(tex_w), ctypes.byref(h)), ctx.h)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and not _EXITING:
+            lib().arh_relief_destroy(self.h)
+            self.h = None
+
+    def eval(self, tex, heights):
+        """heights (device, n) from tex (device, tex_h x tex_w float64); device pointers or arrays."""
+        _check(lib().arh_relief_eval(self.h, _ptr(tex), _ptr(heights)), self.ctx.h)
+
+    def adjoint(self, grad, tex_grad):
+        """tex_grad (device, tex_h x tex_w, overwritten) from grad (device, n)."""
+        _check(lib().arh_relief_adjoint(self.h, _ptr(grad), _ptr(tex_grad)), self.ctx.h)
 
 
 class ListenerACA:

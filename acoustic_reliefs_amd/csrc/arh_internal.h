@@ -321,6 +321,14 @@ struct arh_qaca_s {
     arh::DBuf L, U, V, used, part, dots, st, t;
 };
 
+// Relief parameterisation: taps of the differentiable vertices in a tex_h x tex_w heightfield
+// (four texel indices and weights per vertex) and, per texel, the CSR list of (vertex, tap) entries.
+struct arh_relief_s {
+    arh_ctx *ctx = nullptr;
+    int n = 0, h = 0, w = 0;
+    arh::DBuf idx, wts, ptr, ent;
+};
+
 namespace arh {
 // geometry.hip
 void geom_build_tri(arh_geom *g);
@@ -348,6 +356,10 @@ void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int tr
                       double2 *y);
 void launch_scale(hipStream_t s, double2 *a, int64_t n, double f);
 void launch_diffusion(hipStream_t s, int LL, int nrhs, const double2 *y, double *d, double *u, double2 *vf);
+// relief.hip
+void relief_build(arh_relief *r, const double *uvs);
+void relief_eval(arh_relief *r, const double *tex, double *heights);
+void relief_adjoint(arh_relief *r, const double *grad, double *tex_grad);
 // qaca.hip
 void qaca_build(arh_qaca *q, const double *L_host);
 void qaca_apply(arh_qaca *q, int transpose, int nrhs, const double2 *x, double2 *y);

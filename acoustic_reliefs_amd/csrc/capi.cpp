@@ -520,6 +520,48 @@ void arh_qaca_destroy(arh_qaca *q) {
     geom_release(g);
 }
 
+int arh_relief_create(arh_ctx *ctx, const double *uvs, int n, int tex_h, int tex_w, arh_relief **out) {
+    return guard(ctx, [&] {
+        ARH_CHECK(ctx && uvs && out, ARH_EINVAL, "NULL argument");
+        ARH_CHECK(n >= 1 && tex_h >= 1 && tex_w >= 1, ARH_EINVAL, "n, tex_h and tex_w must be >= 1");
+        ARH_CHECK((int64_t)tex_h * tex_w <= (int64_t)1 << 28 && (int64_t)n <= (int64_t)1 << 28, ARH_EINVAL,
+                  "texture or vertex count too large");
+        set_device(ctx);
+        auto r = std::make_unique<arh_relief>();
+        r->ctx = ctx, r->n = n, r->h = tex_h, r->w = tex_w;
+        relief_build(r.get(), uvs);
+        *out = r.release();
+        ctx->refs++;
+    });
+}
+
+int arh_relief_eval(arh_relief *r, const double *tex, double *heights) {
+    return guard(r ? r->ctx : nullptr, [&] {
+        ARH_CHECK(r && tex && heights, ARH_EINVAL, "NULL argument");
+        set_device(r->ctx);
+        relief_eval(r, tex, heights);
+        ARH_HIP(hipStreamSynchronize(r->ctx->stream));
+    });
+}
+
+int arh_relief_adjoint(arh_relief *r, const double *grad, double *tex_grad) {
+    return guard(r ? r->ctx : nullptr, [&] {
+        ARH_CHECK(r && grad && tex_grad, ARH_EINVAL, "NULL argument");
+        set_device(r->ctx);
+        relief_adjoint(r, grad, tex_grad);
+        ARH_HIP(hipStreamSynchronize(r->ctx->stream));
+    });
+}
+
+void arh_relief_destroy(arh_relief *r) {
+    if (!r) return;
+    arh_ctx *ctx = r->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete r;
+    ctx_release(ctx);
+}
+
 int arh_diffusion(arh_ctx *ctx, int LL, int nrhs, const double *y, double *d, double *u, double *vf) {
     return guard(ctx, [&] {
         ARH_CHECK(ctx && y && d, ARH_EINVAL, "NULL argument");

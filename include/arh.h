@@ -51,6 +51,7 @@ typedef struct arh_geom_s arh_geom;
 typedef struct arh_hmat_s arh_hmat;
 typedef struct arh_server_s arh_server;
 typedef struct arh_qaca_s arh_qaca;
+typedef struct arh_relief_s arh_relief;
 
 typedef struct {
     double tol;        /* relative residual tolerance (BiCGSTAB::setTolerance, adjoint.cpp:93) */
@@ -216,6 +217,18 @@ int arh_qaca_apply(arh_qaca *q, int transpose, int nrhs, const double *x, double
  * either may be NULL. */
 int arh_qaca_factors(const arh_qaca *q, double *U, double *V);
 void arh_qaca_destroy(arh_qaca *q);
+
+/* ------------------------------------------------------------------ relief parameterisation */
+/* Heights of the differentiable vertices sampled from a heightfield texture, and the adjoint: eval_uv
+ * (acoustics_opt.py:24-34) and the texture-gradient scatter of acoustic_gradient (acoustics_opt.py:45-56),
+ * bit for bit, for fixed uvs (host n x 2, in [0, 1]; 0,0 = the texture's first row and column).
+ * arh_relief_eval: heights (device, n) from tex (device, tex_h x tex_w row-major).
+ * arh_relief_adjoint: tex_grad (device, tex_h x tex_w, overwritten) from grad (device, n). Both return when
+ * the result is ready. Keeps a reference to ctx. */
+int arh_relief_create(arh_ctx *ctx, const double *uvs, int n, int tex_h, int tex_w, arh_relief **out);
+int arh_relief_eval(arh_relief *r, const double *tex, double *heights);
+int arh_relief_adjoint(arh_relief *r, const double *grad, double *tex_grad);
+void arh_relief_destroy(arh_relief *r);
 
 /* Diffusion coefficient per rhs from y (device c128, nrhs x LL) with n_freqs = 1:
  * yt = |y|, d = ((sum L)^2 - sum L^2) / ((LL-1) sum L^2), L = yt^2 (adjoint.cpp:112-127, 388-397).
