@@ -35,10 +35,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(grid=64, size=0.6):
+def make_workload(grid=64, size=0.6, obj=""):
+    """The synthetic 0.6 m relief, or (obj) a relief mesh read from an OBJ file (.xz allowed)."""
     from acoustic_reliefs_amd import meshes
-    P, E, top = meshes.relief_config(size=size, grid=grid, hmax=0.1, seed=0)
+    if obj:
+        P, E = read_obj_any(obj)
+    else:
+        P, E, top = meshes.relief_config(size=size, grid=grid, hmax=0.1, seed=0)
     return P, E, meshes.incidence_directions(37), meshes.semicircle_receivers(5.0, 181), meshes.THIRD_OCTAVE_BANDS
+
+
+def read_obj_any(path):
+    import lzma
+    import tempfile
+    from acoustic_reliefs_amd import meshes
+    if not path.endswith(".xz"):
+        return meshes.read_obj(path)
+    with tempfile.TemporaryDirectory() as d:
+        tmp = os.path.join(d, "mesh.obj")
+        with lzma.open(path, "rb") as f, open(tmp, "wb") as g:
+            g.write(f.read())
+        return meshes.read_obj(tmp)
 
 
 def run_mixed(g, bands, dirs, L, capi, lat):
@@ -497,6 +514,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--obj", type=str, default="", help="a relief mesh (OBJ, or .obj.xz) instead of the synthetic one")
     ap.add_argument("--freqs", type=str, default="", help="comma-separated band frequencies (Hz) instead of --bands")
     ap.add_argument("--workers", type=int, default=4, help="server workers (concurrent band solves)")
     ap.add_argument("--sequential", action="store_true", help="solve the bands one after another (no server)")
@@ -545,7 +563,7 @@ def main():
         sys.exit(2)
 
     from acoustic_reliefs_amd import capi
-    P, E, dirs, L, bands = make_workload(args.grid)
+    P, E, dirs, L, bands = make_workload(args.grid, obj=args.obj)
     bands = [float(f) for f in args.freqs.split(",")] if args.freqs else bands[:args.bands]
     ctx = capi.Context(0)
     g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
@@ -595,9 +613,11 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1000.0 * t_mixed / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "c64 operator + c128 refinement",
-           "data": "synthetic",
-           "config": {"workload": "0.6 m relief, %dx%d top (N=%d), %d bands 250-4000 Hz x %d plane waves, "
-                                  "181 receivers at 5 m" % (args.grid, args.grid, g.N, len(bands), len(dirs)),
+           "data": "mesh from " + os.path.basename(args.obj) if args.obj else "synthetic",
+           "config": {"workload": ("relief %s (N=%d)" % (os.path.basename(args.obj), g.N) if args.obj else
+                                   "0.6 m relief, %dx%d top (N=%d)" % (args.grid, args.grid, g.N)) +
+                                  ", %d bands 250-4000 Hz x %d plane waves, 181 receivers at 5 m" % (len(bands),
+                                                                                                     len(dirs)),
                       "solves_per_step": nsolve, "leaf": 32, "radius_factor": 1.5, "aca_tol": 1e-4,
                       "true_residual_tol": 1e-6, "c0": C0,
                       "serving": "sequential bands" if args.sequential else "server, %d workers" % args.workers},
