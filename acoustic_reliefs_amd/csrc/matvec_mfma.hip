@@ -19,8 +19,8 @@
 //    column into the packed [x; t] buffer, so phase 2 (y = D x + U t) is one uniform column sweep.
 // The rhs block and the intermediates share one row-major buffer xt (row i: RP complex values).
 //
-// Work is cut into "groups" of 4 consecutive columns. Each wavefront loads the next kDepth groups
-// (A entries + xt rows) before multiplying the current kDepth, which hides the memory latency
+// Work is cut into "groups" of 4 consecutive columns. Each wavefront loads the next stage of groups
+// (A entries + xt rows) before multiplying the current one, which hides the memory latency
 // behind the MFMAs. Accumulation in the operator's precision; y = -x/2 + A x is formed in FP64.
 //
 // Lane layouts (64 lanes, 16x16x4): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; D[i][j] in
@@ -57,8 +57,11 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int kk, int v) { return v * 4 + kk; }
 };
 
-#ifndef ARH_MV_DEPTH
-#define ARH_MV_DEPTH 2
+#ifndef ARH_MV_DEPTH32
+#define ARH_MV_DEPTH32 2
+#endif
+#ifndef ARH_MV_DEPTH64
+#define ARH_MV_DEPTH64 2
 #endif
 #ifndef ARH_MV_WAVES
 #define ARH_MV_WAVES 8
@@ -69,7 +72,17 @@ struct Mfma<double> {
 #ifndef ARH_MV_XCD
 #define ARH_MV_XCD 1
 #endif
-constexpr int kDepth = ARH_MV_DEPTH;       // groups per pipeline stage of a wavefront
+#ifndef ARH_MV_LDS
+#define ARH_MV_LDS 0
+#endif
+#ifndef ARH_MV_LDS_STAGES
+#define ARH_MV_LDS_STAGES 3
+#endif
+// groups per pipeline stage of a wavefront (FP64 groups hold twice the registers)
+template <class T>
+constexpr int depth_of() {
+    return sizeof(T) == 4 ? ARH_MV_DEPTH32 : ARH_MV_DEPTH64;
+}
 constexpr int kWaves = ARH_MV_WAVES;       // wavefronts of the row-task / big-stack workgroups (power of 2)
 constexpr bool kChunked = ARH_MV_CHUNKED;  // a workgroup's waves take contiguous group ranges (else strided)
 constexpr bool kXcdOrder = ARH_MV_XCD;     // row tasks ordered for the per-die L2s
@@ -77,6 +90,11 @@ constexpr int kXcds = 8;                   // dies (XCDs) of an MI355X, each wit
 constexpr int kBigCols = 256;     // stacks with at least this many columns get a whole workgroup per unit
 constexpr int kPartRows = 64;     // rows of a partial tile (the largest leaf)
 constexpr int kTaskSlots = 1024;  // row tasks aimed for: a few per compute unit
+// FP32 row tasks of leaves <= 32 rows stage operands in LDS (global_load_lds). Off: bit for bit the
+// same results, but 126 us against 93 us for the register path on the 4000 Hz / 37 rhs product (one
+// workgroup per CU fits the rings; the scalar source-row loads and the LDS reads are not yet overlapped).
+constexpr bool kLdsRows = ARH_MV_LDS;
+constexpr int kLdsStages = ARH_MV_LDS_STAGES;  // LDS ring depth (stages of groups) per wavefront
 constexpr int kMaxRC = 40;        // right-hand sides per pass (RP <= 40)
 
 // Right-hand side j of a pass is column cols[j0 + j] of x / y (j0 + j without a list).
@@ -164,35 +182,47 @@ __device__ __forceinline__ void zero_acc(Acc<T, NT, NRT> &acc) {
             for (int v = 0; v < 4; v++) acc[rt][nt][v] = T(0);
 }
 
-// Groups q0, q0 + qstride, ... < qend, kDepth at a time, double-buffered: the next kDepth groups
-// are loaded before the current ones are multiplied, so the only wait on loads comes after kDepth
-// groups of products. A group's uniform metadata (pre: scalar loads) is fetched one stage before
-// its data. Every load is issued unconditionally (indices past the end re-read the last group,
-// whose values are then not used), which keeps the count of loads in flight fixed.
+// Groups q0, q0 + qstride, ... < qend, kDepth at a time, double-buffered as two register sets that
+// swap roles every stage (stage A multiplies set a while set b loads, stage B the other way round), so
+// no register copy has to wait for a load: the only wait is for the set about to be multiplied, one
+// stage after its loads were issued. A group's uniform metadata (pre: scalar loads) is fetched two
+// stages before its data. Every load is issued unconditionally (indices past the end re-read the last
+// group, whose values are then not used), which keeps the count of loads in flight fixed.
 template <class T, int NT, int NRT, bool MASK, class Meta, class Pre, class Load>
 __device__ __forceinline__ void ring_run(Acc<T, NT, NRT> &acc, int q0, int qstride, int qend, int s, Pre &&pre,
                                          Load &&ld) {
+    constexpr int kDepth = depth_of<T>();
     if (q0 >= qend) return;
     const int qlast = qend - 1;
-    Grp<T, NT, NRT> cur[kDepth], nxt[kDepth];
-    Meta mc[kDepth], mn[kDepth];
+    const int step = kDepth * qstride;
+    Grp<T, NT, NRT> ga[kDepth], gb[kDepth];
+    Meta ma[kDepth], mb[kDepth];
 #pragma unroll
-    for (int j = 0; j < kDepth; j++) mc[j] = pre(min(q0 + j * qstride, qlast));
+    for (int j = 0; j < kDepth; j++) ma[j] = pre(min(q0 + j * qstride, qlast));
 #pragma unroll
-    for (int j = 0; j < kDepth; j++) ld(min(q0 + j * qstride, qlast), mc[j], cur[j]);
+    for (int j = 0; j < kDepth; j++) ld(min(q0 + j * qstride, qlast), ma[j], ga[j]);
 #pragma unroll
-    for (int j = 0; j < kDepth; j++) mn[j] = pre(min(q0 + (kDepth + j) * qstride, qlast));
-    for (int qb = q0; qb < qend; qb += kDepth * qstride) {
+    for (int j = 0; j < kDepth; j++) mb[j] = pre(min(q0 + step + j * qstride, qlast));
+    for (int qb = q0; qb < qend; qb += 2 * step) {
+        // stage A: groups qb.. from set a; set b loads groups qb + step..
 #pragma unroll
-        for (int j = 0; j < kDepth; j++) ld(min(qb + (kDepth + j) * qstride, qlast), mn[j], nxt[j]);
+        for (int j = 0; j < kDepth; j++) ld(min(qb + step + j * qstride, qlast), mb[j], gb[j]);
 #pragma unroll
-        for (int j = 0; j < kDepth; j++) mn[j] = pre(min(qb + (2 * kDepth + j) * qstride, qlast));
+        for (int j = 0; j < kDepth; j++) ma[j] = pre(min(qb + 2 * step + j * qstride, qlast));
 #pragma unroll
         for (int j = 0; j < kDepth; j++)
-            if (qb + j * qstride < qend) grp_mma<T, NT, NRT, MASK>(acc, cur[j], s);
+            if (qb + j * qstride < qend) grp_mma<T, NT, NRT, MASK>(acc, ga[j], s);
         __builtin_amdgcn_sched_barrier(0);
+        if (qb + step >= qend) break;
+        // stage B: groups qb + step.. from set b; set a loads groups qb + 2 step..
 #pragma unroll
-        for (int j = 0; j < kDepth; j++) cur[j] = nxt[j];
+        for (int j = 0; j < kDepth; j++) ld(min(qb + 2 * step + j * qstride, qlast), ma[j], ga[j]);
+#pragma unroll
+        for (int j = 0; j < kDepth; j++) mb[j] = pre(min(qb + 3 * step + j * qstride, qlast));
+#pragma unroll
+        for (int j = 0; j < kDepth; j++)
+            if (qb + step + j * qstride < qend) grp_mma<T, NT, NRT, MASK>(acc, gb[j], s);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -312,6 +342,107 @@ __global__ __launch_bounds__(64 * kWaves) void k_lr_mfma(const VStack *__restric
         lr_run<T, NT, 1>(S, base, V, xt, red.v, big, w, lane);
 }
 
+// ---------------------------------------------------------------- LDS-staged operands (FP32 rows)
+// Each wavefront keeps a ring of kLdsStages stages in LDS; a stage holds depth_of<float>() groups, a
+// group slot its A columns (4 x m, column-major, room for 16 NRT rows) followed by its 4 rhs rows
+// (4 x RP). Global -> LDS copies (global_load_lds_dwordx4: lane l writes 16 bytes at slot + 16 l)
+// take no registers, so the loads of the next kLdsStages - 1 stages stay in flight while a stage is
+// multiplied; the only wait is an explicit vmcnt for the stage about to be read. Every copy is
+// issued unconditionally for a clamped group index, which keeps the count per stage fixed.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// wait until at most n vector-memory operations of this wave are outstanding (gfx9 s_waitcnt fields)
+template <int n>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(n >= 0 && n < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int NT, int NRT>
+struct LdsGeom {
+    static constexpr int RP = 8 * NT;
+    static constexpr int ABYTES = 4 * 16 * NRT * 8;           // A columns, float2
+    static constexpr int BBYTES = 4 * RP * 8;                 // rhs rows
+    static constexpr int SLOT = ABYTES + BBYTES;              // bytes of one group
+    static constexpr int NI = (SLOT + 1023) / 1024;           // copies of 64 x 16 bytes per group
+    static constexpr int STAGE = SLOT * 2;                    // two groups per stage (depth_of<float>)
+    static constexpr int WAVE = STAGE * kLdsStages;           // one wave's ring
+};
+
+// Issue the copies of one group into slot (LDS byte address): A = 4 m float2 from acol, rows r[k] of xt.
+template <int NT, int NRT>
+__device__ __forceinline__ void lds_issue(char *slot, const float2 *acol, int m, const int4 &r,
+                                          const float2 *__restrict__ xt, int lane) {
+    typedef LdsGeom<NT, NRT> G;
+    const int abytes = 32 * m;
+#pragma unroll
+    for (int i = 0; i < G::NI; i++) {
+        const int o = i * 1024 + 16 * lane;
+        const char *src = nullptr;
+        if (o < abytes) {
+            src = reinterpret_cast<const char *>(acol) + o;
+        } else if (o >= G::ABYTES && o < G::SLOT) {
+            const int j = o - G::ABYTES, k = j / (G::RP * 8), w = j - k * (G::RP * 8);
+            const int row = k == 0 ? r.x : k == 1 ? r.y : k == 2 ? r.z : r.w;
+            src = reinterpret_cast<const char *>(xt + (int64_t)row * G::RP) + w;
+        }
+        if (src) __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(slot + i * 1024), 16, 0, 0);
+    }
+}
+
+// One group's MFMAs from its LDS slot (same products and order as grp_mma).
+template <int NT, int NRT>
+__device__ __forceinline__ void lds_mma(Acc<float, NT, NRT> &acc, const char *slot, int m, int lane) {
+    typedef LdsGeom<NT, NRT> G;
+    const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
+    const float2 *A = reinterpret_cast<const float2 *>(slot) + kk * m;
+    const float2 *B = reinterpret_cast<const float2 *>(slot + G::ABYTES) + kk * G::RP + (i16 >> 1);
+    Grp<float, NT, NRT> g;
+    g.okf = 1.0f;
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) g.a[rt] = A[min(rt * 16 + i16, m - 1)];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) g.xv[nt] = B[nt * 8];
+    grp_mma<float, NT, NRT, false>(acc, g, s);
+}
+
+// The groups q0, q0 + qstride, ... < qend of a row task from the LDS ring. Groups are taken two per
+// stage in the order ring_run takes them, so the products are the same in the same order (the
+// results are bit for bit those of the register path).
+template <int NT, int NRT>
+__device__ __forceinline__ void lds_ring_run(Acc<float, NT, NRT> &acc, int q0, int qstride, int qend, char *ring,
+                                             const float2 *A0, int m, const int4 *src4,
+                                             const float2 *__restrict__ xt, int lane) {
+    typedef LdsGeom<NT, NRT> G;
+    constexpr int D = 2, S = kLdsStages;
+    static_assert(depth_of<float>() == D, "LDS ring stages hold depth_of<float>() groups");
+    if (q0 >= qend) return;
+    const int qlast = qend - 1;
+    const int nst = ((qend - q0 + qstride - 1) / qstride + D - 1) / D;  // stages of this wave
+    auto issue = [&](int st) {
+        char *base = ring + (st % S) * G::STAGE;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            const int q = min(q0 + (st * D + j) * qstride, qlast);
+            const int4 r = src4[q];
+            lds_issue<NT, NRT>(base + j * G::SLOT, A0 + (int64_t)q * 4 * m, m, r, xt, lane);
+        }
+    };
+#pragma unroll
+    for (int st = 0; st < S - 1; st++) issue(st);
+    for (int st = 0; st < nst; st++) {
+        issue(st + S - 1);
+        wait_vmcnt<(S - 1) * D * G::NI>();
+        __builtin_amdgcn_sched_barrier(0);
+        const char *base = ring + (st % S) * G::STAGE;
+#pragma unroll
+        for (int j = 0; j < D; j++)
+            if (q0 + (st * D + j) * qstride < qend) lds_mma<NT, NRT>(acc, base + j * G::SLOT, m, lane);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    wait_vmcnt<0>();  // the ring's last copies land before the LDS is reused
+}
+
 // ---------------------------------------------------------------- phase 2: row-leaf tasks
 // A task is a range of one row leaf's panel groups, taken by one workgroup: each wave takes a share
 // of the groups (wave_share) and the partial tiles are summed in a fixed order in LDS. A leaf with
@@ -332,6 +463,11 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
     const T2 *A0 = panel + L.off + (int64_t)kk * m;
     const int4 *src4 = reinterpret_cast<const int4 *>(psrc + L.col0) + T_.g0;
     const int3 sh = wave_share(T_.g1 - T_.g0, w);
+    if constexpr (kLdsRows && sizeof(T) == 4 && NRT <= 2) {
+        char *ring = reinterpret_cast<char *>(red) + w * LdsGeom<NT, NRT>::WAVE;
+        lds_ring_run<NT, NRT>(acc, sh.x, sh.y, sh.z, ring, panel + L.off + (int64_t)T_.g0 * 4 * m, m, src4, xt, lane);
+        __syncthreads();  // every wave is done with its ring before the reduction reuses the LDS
+    } else
     ring_run<T, NT, NRT, false, int4>(
         acc, sh.x, sh.y, sh.z, s, [&](int q) { return src4[q]; },  // the group's 4 source rows (scalar load)
         [&](int q, const int4 &sr, Grp<T, NT, NRT> &g) {
@@ -361,7 +497,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(
     const int *__restrict__ psrc, const typename Mfma<T>::T2 *__restrict__ panel,
     const typename Mfma<T>::T2 *__restrict__ xt, const double *__restrict__ x, int N, int R, double *__restrict__ y,
     const int *__restrict__ cols, int j0, const int *__restrict__ gate, T *__restrict__ part) {
-    __shared__ Lds<T, NT> red;
+    // the reduction buffer, or (LDS-staged FP32 rows) the wavefronts' rings, whichever is larger
+    constexpr bool lds_rows = kLdsRows && sizeof(T) == 4 && MAXRT == 2;
+    constexpr size_t ring_bytes = lds_rows ? (size_t)kWaves * LdsGeom<NT, 2>::WAVE : 0;
+    constexpr size_t bytes = std::max(sizeof(Lds<T, NT>), ring_bytes);
+    __shared__ __attribute__((aligned(16))) char smem[bytes];
+    Lds<T, NT> &red = *reinterpret_cast<Lds<T, NT> *>(smem);
     if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const RowTask T_ = tasks[blockIdx.x];
