@@ -164,6 +164,22 @@ def cpu_baseline(P, E, dirs, L, threads=None, budget_s=20.0):
                           threads, t_asm, n, t_solve)}
 
 
+def measured_traffic(kernel="k_rows_mfma<float, 5, 2>"):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE passes, each scaled by its reading for a plain device copy); None when absent."""
+    path = os.path.join(ROOT, "profiles", "k_rows_mfma_relief0.6m_4000Hz_37rhs.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == "`%s`" % kernel:
+            try:
+                return float(cells[3]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def isolated_roofline(g, capi, ctx, f=4000.0, reps=20):
     """The dominant kernel on its own stream, nothing else in flight: the 37-rhs FP32 MFMA product of
     the f Hz band's operator (and the FP64 one), HIP-event timed."""
@@ -190,7 +206,10 @@ def isolated_roofline(g, capi, ctx, f=4000.0, reps=20):
         out[name] = {"kernel": "k_rows_mfma<%s, 5, 2> (%g Hz band, %d rhs, N=%d, %d intermediates rows)" % (
                          "float" if name == "fp32" else "double", f, R, N, T),
                      "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                     "traffic": None, "avg_launch_us": 1000 * rw["ms"] / rw["launches"], "launches": rw["launches"],
+                     "traffic": measured_traffic("k_rows_mfma<%s, 5, 2>" % ("float" if name == "fp32" else "double")),
+                     "traffic_source": "profiles/k_rows_mfma_relief0.6m_4000Hz_37rhs.md (rocprofv3 FETCH_SIZE x 2.000 "
+                                       "+ WRITE_SIZE x 1.000 per dispatch, same kernel and workload)",
+                     "avg_launch_us": 1000 * rw["ms"] / rw["launches"], "launches": rw["launches"],
                      "flops_per_launch": rw["flops"] / rw["launches"], "bytes_per_launch": rw["bytes"] / rw["launches"],
                      "lowrank_phase_us": 1000 * lr["ms"] / max(1, lr["launches"]),
                      "hbm_GBs": rw["bytes"] / (rw["ms"] * 1e-3) / 1e9,
