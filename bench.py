@@ -253,6 +253,7 @@ def run_gradient(args):
         log("engine (partition, geometry) %.2f s" % (time.perf_counter() - t_init))
         capi.device_trim(0)  # blocks kept for reuse are not held by the engine
         free0 = torch.cuda.mem_get_info()[0]
+        ctx.profile(True)  # HIP-event totals of the H-matrix product kernels over forward + backward
         h = torch.zeros(len(top), dtype=torch.float64, requires_grad=True)
         t0 = time.perf_counter()
         d = autograd.diffusion_coefficients(h, eng)
@@ -264,8 +265,20 @@ def run_gradient(args):
         loss.backward()
         ctx.synchronize()
         t2 = time.perf_counter()
+        prof = ctx.get_profile()
+        ctx.profile(False)
+        kern = {}
+        for name, peak in (("rows_fp32", FP32_PEAK_TFLOPS), ("lowrank_fp32", FP32_PEAK_TFLOPS),
+                           ("rows_fp64", FP64_PEAK_TFLOPS), ("lowrank_fp64", FP64_PEAK_TFLOPS)):
+            kp = prof[name]
+            if kp["launches"] and kp["ms"] > 0:
+                tf = kp["flops"] / (kp["ms"] * 1e-3) / 1e12
+                kern[name] = {"launches": kp["launches"], "ms": kp["ms"], "TFLOPs": tf, "mfma_frac": tf / peak,
+                              "GBs": kp["bytes"] / (kp["ms"] * 1e-3) / 1e9,
+                              "hbm_frac": kp["bytes"] / (kp["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
         key = "recompute" if recompute else "stored"
         out[key] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "loss": float(loss.detach()),
+                    "product_kernels": kern,
                     "device_bytes_held_after_forward": free0 - free1,
                     "hmat_bytes_per_band": [b["hmat_bytes"] for b in eng.timing],
                     "per_band": eng.timing}
