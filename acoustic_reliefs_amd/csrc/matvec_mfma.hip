@@ -75,6 +75,9 @@ struct Mfma<double> {
 #ifndef ARH_MV_LDS
 #define ARH_MV_LDS 0
 #endif
+#ifndef ARH_MV_SLOTS
+#define ARH_MV_SLOTS 1024
+#endif
 #ifndef ARH_MV_LDS_STAGES
 #define ARH_MV_LDS_STAGES 3
 #endif
@@ -89,7 +92,7 @@ constexpr bool kXcdOrder = ARH_MV_XCD;     // row tasks ordered for the per-die 
 constexpr int kXcds = 8;                   // dies (XCDs) of an MI355X, each with its own L2
 constexpr int kBigCols = 256;     // stacks with at least this many columns get a whole workgroup per unit
 constexpr int kPartRows = 64;     // rows of a partial tile (the largest leaf)
-constexpr int kTaskSlots = 1024;  // row tasks aimed for: a few per compute unit
+constexpr int kTaskSlots = ARH_MV_SLOTS;  // row tasks aimed for: a few per compute unit
 // FP32 row tasks of leaves <= 32 rows stage operands in LDS (global_load_lds). Off: bit for bit the
 // same results, but 126 us against 93 us for the register path on the 4000 Hz / 37 rhs product (one
 // workgroup per CU fits the rings; the scalar source-row loads and the LDS reads are not yet overlapped).
@@ -117,18 +120,31 @@ __global__ void k_pack_xf(const double2 *__restrict__ x, int N, int R, int RP, t
     }
 }
 
-// one B operand pair from a packed complex rhs row: output column n = 16 nt + i16 -> rhs n / 2, part n % 2
-template <class T2, class T>
-__device__ __forceinline__ void b_pair(T2 xv, int s, T &br, T &bi) {
-    br = s ? xv.y : xv.x;
-    bi = s ? xv.x : -xv.y;
+// The value of lane l ^ 1 (DPP quad permutation [1, 0, 3, 2]; every lane of the wavefront is active).
+__device__ __forceinline__ float pair_swap(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-// A group in registers: A entries of NRT 16-row tiles, rhs rows for NT output tiles.
+// The B operands of output column n = 16 nt + i16 (rhs n / 2, part s = n % 2) from the one real
+// value the lane holds, xb = component s of that rhs's xt entry (the lane pair n, n ^ 1 holds its
+// real and imaginary parts): br = xb, bi = the partner's value, negated on the real-part lane.
+template <class T>
+__device__ __forceinline__ void b_pair(T xb, int s, T &br, T &bi) {
+    const T nb = pair_swap(xb);
+    br = xb;
+    bi = s ? nb : -nb;
+}
+
+// A group in registers: A entries of NRT 16-row tiles, one real rhs value per output tile.
 template <class T, int NT, int NRT>
 struct Grp {
     typename Mfma<T>::T2 a[NRT];
-    typename Mfma<T>::T2 xv[NT];
+    T xb[NT];
     T okf;  // 0 for a padding column (masked groups only)
 };
 
@@ -141,7 +157,7 @@ template <class T, int NT, int NRT, bool MASK>
 __device__ __forceinline__ void grp_mma(Acc<T, NT, NRT> &acc, const Grp<T, NT, NRT> &g, int s) {
     T br[NT], bi[NT], ar[NRT], ai[NRT];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) b_pair(g.xv[nt], s, br[nt], bi[nt]);
+    for (int nt = 0; nt < NT; nt++) b_pair(g.xb[nt], s, br[nt], bi[nt]);
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++) {
         ar[rt] = MASK ? g.a[rt].x * g.okf : g.a[rt].x;
@@ -158,16 +174,16 @@ __device__ __forceinline__ void grp_mma(Acc<T, NT, NRT> &acc, const Grp<T, NT, N
 }
 
 // Load one column of a group: A column at acol (rows past amax clamped: their outputs are
-// dropped), xt row at brow. No load is conditional, and nothing waits on a load before the group
+// dropped), xt row at brow (real values: the lane's output column is brow[16 nt]). No load is conditional, and nothing waits on a load before the group
 // is consumed (a padding mask is applied then).
 template <class T, int NT, int NRT>
 __device__ __forceinline__ void grp_load(Grp<T, NT, NRT> &g, T okf, const typename Mfma<T>::T2 *acol, int amax,
-                                         int i16, const typename Mfma<T>::T2 *brow) {
+                                         int i16, const T *brow) {
     g.okf = okf;
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++) g.a[rt] = acol[min(rt * 16 + i16, amax)];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) g.xv[nt] = brow[nt * 8];
+    for (int nt = 0; nt < NT; nt++) g.xb[nt] = brow[nt * 16];
 }
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
@@ -235,13 +251,12 @@ __device__ __forceinline__ int3 wave_share(int n, int w) {
     return make_int3(w, kWaves, n);
 }
 
-// LDS of a reduction / output staging: kWaves / 2 slots of NT * 256 values (one row tile each),
-// which is also 64 rows x 16 NT values of an output tile.
+// LDS of a reduction / output staging: kWaves / 2 slots of NT * 256 values (one row tile each), and
+// room for kPartRows x 16 NT values of an output tile.
 template <class T, int NT>
 struct Lds {
-    T v[(kWaves / 2) * NT * 256];
+    T v[std::max((kWaves / 2) * NT * 256, kPartRows * 16 * NT)];
 };
-static_assert((kWaves / 2) * 256 >= kPartRows * 16, "output staging must fit the reduction buffer");
 
 // Fixed-order tree reduction of the kWaves wave partials into wave 0, one row tile at a time:
 // (w, w + kWaves/2), ..., (0, 1).
@@ -300,7 +315,7 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const typename
     const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
     const int amax = min(32, r - base) - 1;
     const T2 *A0 = V + S.v_off + base;
-    const T2 *B0 = xt + (int64_t)S.cs * RP + (i16 >> 1);
+    const T *B0 = reinterpret_cast<const T *>(xt) + (int64_t)S.cs * 2 * RP + i16;
     Acc<T, NT, NRT> acc;
     zero_acc<T, NT, NRT>(acc);
     const int ngr = (n + 3) >> 2;
@@ -311,7 +326,7 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const typename
             const int c = q * 4 + kk;
             const bool ok = c < n;
             const int cc = ok ? c : q * 4;
-            grp_load<T, NT, NRT>(g, ok ? T(1) : T(0), A0 + (int64_t)cc * r, amax, i16, B0 + (int64_t)cc * RP);
+            grp_load<T, NT, NRT>(g, ok ? T(1) : T(0), A0 + (int64_t)cc * r, amax, i16, B0 + (int64_t)cc * 2 * RP);
         });
     if (big) {
         reduce_waves<T, NT, NRT>(acc, red, w, lane);
@@ -396,13 +411,13 @@ __device__ __forceinline__ void lds_mma(Acc<float, NT, NRT> &acc, const char *sl
     typedef LdsGeom<NT, NRT> G;
     const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
     const float2 *A = reinterpret_cast<const float2 *>(slot) + kk * m;
-    const float2 *B = reinterpret_cast<const float2 *>(slot + G::ABYTES) + kk * G::RP + (i16 >> 1);
+    const float *B = reinterpret_cast<const float *>(slot + G::ABYTES) + kk * 2 * G::RP + i16;
     Grp<float, NT, NRT> g;
     g.okf = 1.0f;
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++) g.a[rt] = A[min(rt * 16 + i16, m - 1)];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) g.xv[nt] = B[nt * 8];
+    for (int nt = 0; nt < NT; nt++) g.xb[nt] = B[nt * 16];
     grp_mma<float, NT, NRT, false>(acc, g, s);
 }
 
@@ -473,7 +488,7 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
         [&](int q, const int4 &sr, Grp<T, NT, NRT> &g) {
             const int row = kk == 0 ? sr.x : kk == 1 ? sr.y : kk == 2 ? sr.z : sr.w;
             grp_load<T, NT, NRT>(g, T(1), A0 + (int64_t)(T_.g0 + q) * 4 * m, m - 1, i16,
-                                 xt + (int64_t)row * RP + (i16 >> 1));
+                                 reinterpret_cast<const T *>(xt) + (int64_t)row * 2 * RP + i16);
         });
     reduce_waves<T, NT, NRT>(acc, red, w, lane);
     __syncthreads();  // every wave has read the reduction buffer before it is reused for staging
@@ -522,28 +537,45 @@ __global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(
     }
 }
 
-// y = -x/2 + the task partials of each split leaf, summed in task order: partial rows are read
-// contiguously into LDS, y rows are written contiguously from it.
+// y = -x/2 + the task partials of each split leaf, summed in task order. Workgroup (leaf, chunk) takes
+// kSumRows rows of one split leaf: every thread holds its few elements in registers and issues their
+// loads of one partial together, so a workgroup waits once per partial rather than once per element;
+// the summed rows go through LDS so that y rows are written contiguously.
+constexpr int kSumRows = 8;
 template <class T, int NT>
 __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split, const int *__restrict__ leaf_start,
                                                   const T *__restrict__ part, const double *__restrict__ x, int N,
                                                   int R, double *__restrict__ y, const int *__restrict__ cols, int j0,
                                                   const int *__restrict__ gate) {
-    constexpr int W = 16 * NT;
-    __shared__ T tile[kPartRows * W];
+    constexpr int W = 16 * NT, E = kSumRows * W, EPT = (E + 255) / 256;
+    __shared__ T tile[E];
     if (gated(gate)) return;
     const int4 S = split[blockIdx.x];
     const int rs = leaf_start[S.x], m = leaf_start[S.x + 1] - rs;
-    const T *src = part + (int64_t)S.y * kPartRows * W;
-    for (int e = threadIdx.x; e < m * W; e += blockDim.x) {
-        T acc = T(0);
-        for (int k = 0; k < S.z; k++) acc += src[(int64_t)k * kPartRows * W + e];
-        tile[e] = acc;
+    const int r0 = (int)blockIdx.y * kSumRows;
+    if (r0 >= m) return;
+    const int mr = min(kSumRows, m - r0), ne = mr * W;
+    const T *src = part + (int64_t)S.y * kPartRows * W + (int64_t)r0 * W;
+    T acc[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) acc[i] = T(0);
+    for (int k = 0; k < S.z; k++) {
+        const T *sk = src + (int64_t)k * kPartRows * W;
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            const int e = (int)threadIdx.x + i * 256;
+            if (e < ne) acc[i] += sk[e];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int e = (int)threadIdx.x + i * 256;
+        if (e < ne) tile[e] = acc[i];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 2 * m * R; e += blockDim.x) {
-        const int j = e / (2 * m), q = e - j * 2 * m, row = q >> 1, p = q & 1;
-        const int64_t idx = (col_of(cols, j0, j) * N + rs + row) * 2 + p;
+    for (int e = threadIdx.x; e < 2 * mr * R; e += blockDim.x) {
+        const int j = e / (2 * mr), q = e - j * 2 * mr, row = q >> 1, p = q & 1;
+        const int64_t idx = (col_of(cols, j0, j) * N + rs + r0 + row) * 2 + p;
         y[idx] = -0.5 * x[idx] + (double)tile[row * W + 2 * j + p];
     }
 }
@@ -850,8 +882,8 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
         }
     });
     if (nsp)
-        hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp), dim3(256), 0, s, M.d_rsplit.as<int4>(),
-                           g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
+        hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp, (g->max_leaf + kSumRows - 1) / kSumRows), dim3(256), 0, s,
+                           M.d_rsplit.as<int4>(), g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
     ARH_HIP(hipGetLastError());
 }
 
