@@ -63,8 +63,17 @@ struct Mfma<double> {
 #ifndef ARH_MV_DEPTH64
 #define ARH_MV_DEPTH64 2
 #endif
+#ifndef ARH_MV_RDEPTH32
+#define ARH_MV_RDEPTH32 3
+#endif
+#ifndef ARH_MV_RDEPTH64
+#define ARH_MV_RDEPTH64 2
+#endif
 #ifndef ARH_MV_WAVES
 #define ARH_MV_WAVES 8
+#endif
+#ifndef ARH_MV_RWAVES
+#define ARH_MV_RWAVES 4
 #endif
 #ifndef ARH_MV_CHUNKED
 #define ARH_MV_CHUNKED 1
@@ -76,17 +85,25 @@ struct Mfma<double> {
 #define ARH_MV_LDS 0
 #endif
 #ifndef ARH_MV_SLOTS
-#define ARH_MV_SLOTS 1024
+#define ARH_MV_SLOTS 2048
 #endif
 #ifndef ARH_MV_LDS_STAGES
 #define ARH_MV_LDS_STAGES 3
 #endif
-// groups per pipeline stage of a wavefront (FP64 groups hold twice the registers)
+// Groups per pipeline stage of a wavefront, for the V-stack (lr) and the row-task kernels (FP64 groups hold
+// twice the registers).
 template <class T>
-constexpr int depth_of() {
+constexpr int lr_depth() {
     return sizeof(T) == 4 ? ARH_MV_DEPTH32 : ARH_MV_DEPTH64;
 }
-constexpr int kWaves = ARH_MV_WAVES;       // wavefronts of the row-task / big-stack workgroups (power of 2)
+template <class T>
+constexpr int row_depth() {
+    return sizeof(T) == 4 ? ARH_MV_RDEPTH32 : ARH_MV_RDEPTH64;
+}
+// Wavefronts per workgroup (powers of 2). Row tasks take 4: a workgroup then holds one wavefront per SIMD,
+// so the register budget, not the workgroup size, sets how many fit a compute unit.
+constexpr int kLrWaves = ARH_MV_WAVES;    // V-stack units (big stacks: the whole workgroup)
+constexpr int kRowWaves = ARH_MV_RWAVES;  // row tasks
 constexpr bool kChunked = ARH_MV_CHUNKED;  // a workgroup's waves take contiguous group ranges (else strided)
 constexpr bool kXcdOrder = ARH_MV_XCD;     // row tasks ordered for the per-die L2s
 constexpr int kXcds = 8;                   // dies (XCDs) of an MI355X, each with its own L2
@@ -186,6 +203,13 @@ __device__ __forceinline__ void grp_load(Grp<T, NT, NRT> &g, T okf, const typena
     for (int nt = 0; nt < NT; nt++) g.xb[nt] = brow[nt * 16];
 }
 
+// Component kk (0..3) of v as bit selects (lane-dependent kk: no branches, no exec-mask changes).
+__device__ __forceinline__ int pick4(const int4 &v, int kk) {
+    const int m1 = -(kk & 1), m2 = -((kk >> 1) & 1);
+    const int lo = (v.y & m1) | (v.x & ~m1), hi = (v.w & m1) | (v.z & ~m1);
+    return (hi & m2) | (lo & ~m2);
+}
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 template <class T, int NT, int NRT>
@@ -198,16 +222,26 @@ __device__ __forceinline__ void zero_acc(Acc<T, NT, NRT> &acc) {
             for (int v = 0; v < 4; v++) acc[rt][nt][v] = T(0);
 }
 
+// An empty use of a group's registers. Placed on the loop's exit path it makes the loaded values live
+// there too, so the compiler cannot sink a stage's loads past the loop's branches to just before the
+// products that read them (which would leave no load in flight while a wavefront multiplies).
+template <class T, int NT, int NRT>
+__device__ __forceinline__ void keep(const Grp<T, NT, NRT> &g) {
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) asm volatile("" ::"v"(g.a[rt].x), "v"(g.a[rt].y));
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) asm volatile("" ::"v"(g.xb[nt]));
+}
+
 // Groups q0, q0 + qstride, ... < qend, kDepth at a time, double-buffered as two register sets that
 // swap roles every stage (stage A multiplies set a while set b loads, stage B the other way round), so
 // no register copy has to wait for a load: the only wait is for the set about to be multiplied, one
 // stage after its loads were issued. A group's uniform metadata (pre: scalar loads) is fetched two
 // stages before its data. Every load is issued unconditionally (indices past the end re-read the last
 // group, whose values are then not used), which keeps the count of loads in flight fixed.
-template <class T, int NT, int NRT, bool MASK, class Meta, class Pre, class Load>
+template <class T, int NT, int NRT, bool MASK, int kDepth, class Meta, class Pre, class Load>
 __device__ __forceinline__ void ring_run(Acc<T, NT, NRT> &acc, int q0, int qstride, int qend, int s, Pre &&pre,
                                          Load &&ld) {
-    constexpr int kDepth = depth_of<T>();
     if (q0 >= qend) return;
     const int qlast = qend - 1;
     const int step = kDepth * qstride;
@@ -225,9 +259,13 @@ __device__ __forceinline__ void ring_run(Acc<T, NT, NRT> &acc, int q0, int qstri
         for (int j = 0; j < kDepth; j++) ld(min(qb + step + j * qstride, qlast), mb[j], gb[j]);
 #pragma unroll
         for (int j = 0; j < kDepth; j++) ma[j] = pre(min(qb + 2 * step + j * qstride, qlast));
+        __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products that wait for the other set
 #pragma unroll
         for (int j = 0; j < kDepth; j++)
-            if (qb + j * qstride < qend) grp_mma<T, NT, NRT, MASK>(acc, ga[j], s);
+            if (qb + j * qstride < qend)
+                grp_mma<T, NT, NRT, MASK>(acc, ga[j], s);
+            else
+                keep(ga[j]);  // a skipped group's loads are waited for here, not at the next loop head
         __builtin_amdgcn_sched_barrier(0);
         if (qb + step >= qend) break;
         // stage B: groups qb + step.. from set b; set a loads groups qb + 2 step..
@@ -235,37 +273,44 @@ __device__ __forceinline__ void ring_run(Acc<T, NT, NRT> &acc, int q0, int qstri
         for (int j = 0; j < kDepth; j++) ld(min(qb + 2 * step + j * qstride, qlast), ma[j], ga[j]);
 #pragma unroll
         for (int j = 0; j < kDepth; j++) mb[j] = pre(min(qb + 3 * step + j * qstride, qlast));
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < kDepth; j++)
-            if (qb + step + j * qstride < qend) grp_mma<T, NT, NRT, MASK>(acc, gb[j], s);
+            if (qb + step + j * qstride < qend)
+                grp_mma<T, NT, NRT, MASK>(acc, gb[j], s);
+            else
+                keep(gb[j]);
         __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int j = 0; j < kDepth; j++) keep(ga[j]), keep(gb[j]);
 }
 
-// The groups of [0, n) that wave w of a kWaves workgroup takes: (first, stride, end).
+// The groups of [0, n) that wave w of a W-wave workgroup takes: (first, stride, end).
+template <int W>
 __device__ __forceinline__ int3 wave_share(int n, int w) {
     if (kChunked) {
-        const int chunk = (n + kWaves - 1) / kWaves;
+        const int chunk = (n + W - 1) / W;
         return make_int3(w * chunk, 1, min(n, (w + 1) * chunk));
     }
-    return make_int3(w, kWaves, n);
+    return make_int3(w, W, n);
 }
 
-// LDS of a reduction / output staging: kWaves / 2 slots of NT * 256 values (one row tile each), and
-// room for kPartRows x 16 NT values of an output tile.
-template <class T, int NT>
+// LDS of a reduction / output staging: W / 2 slots of NT * 256 values (one row tile each), and room for
+// kPartRows x 16 NT values of an output tile.
+template <class T, int NT, int W>
 struct Lds {
-    T v[std::max((kWaves / 2) * NT * 256, kPartRows * 16 * NT)];
+    T v[std::max((W / 2) * NT * 256, kPartRows * 16 * NT)];
 };
 
-// Fixed-order tree reduction of the kWaves wave partials into wave 0, one row tile at a time:
-// (w, w + kWaves/2), ..., (0, 1).
-template <class T, int NT, int NRT>
+// Fixed-order tree reduction of the W wave partials into wave 0, one row tile at a time:
+// (w, w + W/2), ..., (0, 1).
+template <int W, class T, int NT, int NRT>
 __device__ __forceinline__ void reduce_waves(Acc<T, NT, NRT> &acc, T *red, int w, int lane) {
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++) {
 #pragma unroll
-        for (int half = kWaves / 2; half >= 1; half >>= 1) {
+        for (int half = W / 2; half >= 1; half >>= 1) {
             __syncthreads();
             if (w >= half && w < 2 * half) {
                 T *r = red + (w - half) * NT * 256;
@@ -319,8 +364,8 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const typename
     Acc<T, NT, NRT> acc;
     zero_acc<T, NT, NRT>(acc);
     const int ngr = (n + 3) >> 2;
-    const int3 sh = big ? wave_share(ngr, w) : make_int3(0, 1, ngr);
-    ring_run<T, NT, NRT, true, int>(
+    const int3 sh = big ? wave_share<kLrWaves>(ngr, w) : make_int3(0, 1, ngr);
+    ring_run<T, NT, NRT, true, lr_depth<T>(), int>(
         acc, sh.x, sh.y, sh.z, s, [](int) { return 0; },
         [&](int q, int, Grp<T, NT, NRT> &g) {
             const int c = q * 4 + kk;
@@ -329,7 +374,7 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const typename
             grp_load<T, NT, NRT>(g, ok ? T(1) : T(0), A0 + (int64_t)cc * r, amax, i16, B0 + (int64_t)cc * 2 * RP);
         });
     if (big) {
-        reduce_waves<T, NT, NRT>(acc, red, w, lane);
+        reduce_waves<kLrWaves, T, NT, NRT>(acc, red, w, lane);
         if (w != 0) return;
     }
     T *out = reinterpret_cast<T *>(xt + (S.t_row + base) * RP);
@@ -337,16 +382,16 @@ __device__ __forceinline__ void lr_run(const VStack &S, int base, const typename
 }
 
 template <class T, int NT>
-__global__ __launch_bounds__(64 * kWaves) void k_lr_mfma(const VStack *__restrict__ stacks,
+__global__ __launch_bounds__(64 * kLrWaves) void k_lr_mfma(const VStack *__restrict__ stacks,
                                                          const int2 *__restrict__ units, int nbig, int nunits,
                                                          const typename Mfma<T>::T2 *__restrict__ V,
                                                          typename Mfma<T>::T2 *__restrict__ xt,
                                                          const int *__restrict__ gate) {
-    __shared__ Lds<T, NT> red;
+    __shared__ Lds<T, NT, kLrWaves> red;
     if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const bool big = (int)blockIdx.x < nbig;
-    const int u = big ? (int)blockIdx.x : nbig + ((int)blockIdx.x - nbig) * kWaves + w;
+    const int u = big ? (int)blockIdx.x : nbig + ((int)blockIdx.x - nbig) * kLrWaves + w;
     if (u >= nunits) return;  // small units only: big workgroups always have a unit
     const int2 un = units[u];
     const VStack S = stacks[un.x];
@@ -358,7 +403,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_lr_mfma(const VStack *__restric
 }
 
 // ---------------------------------------------------------------- LDS-staged operands (FP32 rows)
-// Each wavefront keeps a ring of kLdsStages stages in LDS; a stage holds depth_of<float>() groups, a
+// Each wavefront keeps a ring of kLdsStages stages in LDS; a stage holds row_depth<float>() groups, a
 // group slot its A columns (4 x m, column-major, room for 16 NRT rows) followed by its 4 rhs rows
 // (4 x RP). Global -> LDS copies (global_load_lds_dwordx4: lane l writes 16 bytes at slot + 16 l)
 // take no registers, so the loads of the next kLdsStages - 1 stages stay in flight while a stage is
@@ -380,7 +425,7 @@ struct LdsGeom {
     static constexpr int BBYTES = 4 * RP * 8;                 // rhs rows
     static constexpr int SLOT = ABYTES + BBYTES;              // bytes of one group
     static constexpr int NI = (SLOT + 1023) / 1024;           // copies of 64 x 16 bytes per group
-    static constexpr int STAGE = SLOT * 2;                    // two groups per stage (depth_of<float>)
+    static constexpr int STAGE = SLOT * row_depth<float>();   // groups per stage as in the register path
     static constexpr int WAVE = STAGE * kLdsStages;           // one wave's ring
 };
 
@@ -429,8 +474,7 @@ __device__ __forceinline__ void lds_ring_run(Acc<float, NT, NRT> &acc, int q0, i
                                              const float2 *A0, int m, const int4 *src4,
                                              const float2 *__restrict__ xt, int lane) {
     typedef LdsGeom<NT, NRT> G;
-    constexpr int D = 2, S = kLdsStages;
-    static_assert(depth_of<float>() == D, "LDS ring stages hold depth_of<float>() groups");
+    constexpr int D = row_depth<float>(), S = kLdsStages;
     if (q0 >= qend) return;
     const int qlast = qend - 1;
     const int nst = ((qend - q0 + qstride - 1) / qstride + D - 1) / D;  // stages of this wave
@@ -477,20 +521,20 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
     zero_acc<T, NT, NRT>(acc);
     const T2 *A0 = panel + L.off + (int64_t)kk * m;
     const int4 *src4 = reinterpret_cast<const int4 *>(psrc + L.col0) + T_.g0;
-    const int3 sh = wave_share(T_.g1 - T_.g0, w);
+    const int3 sh = wave_share<kRowWaves>(T_.g1 - T_.g0, w);
     if constexpr (kLdsRows && sizeof(T) == 4 && NRT <= 2) {
         char *ring = reinterpret_cast<char *>(red) + w * LdsGeom<NT, NRT>::WAVE;
         lds_ring_run<NT, NRT>(acc, sh.x, sh.y, sh.z, ring, panel + L.off + (int64_t)T_.g0 * 4 * m, m, src4, xt, lane);
         __syncthreads();  // every wave is done with its ring before the reduction reuses the LDS
     } else
-    ring_run<T, NT, NRT, false, int4>(
+    ring_run<T, NT, NRT, false, row_depth<T>(), int4>(
         acc, sh.x, sh.y, sh.z, s, [&](int q) { return src4[q]; },  // the group's 4 source rows (scalar load)
         [&](int q, const int4 &sr, Grp<T, NT, NRT> &g) {
-            const int row = kk == 0 ? sr.x : kk == 1 ? sr.y : kk == 2 ? sr.z : sr.w;
+            const int row = pick4(sr, kk);
             grp_load<T, NT, NRT>(g, T(1), A0 + (int64_t)(T_.g0 + q) * 4 * m, m - 1, i16,
                                  reinterpret_cast<const T *>(xt) + (int64_t)row * 2 * RP + i16);
         });
-    reduce_waves<T, NT, NRT>(acc, red, w, lane);
+    reduce_waves<kRowWaves, T, NT, NRT>(acc, red, w, lane);
     __syncthreads();  // every wave has read the reduction buffer before it is reused for staging
     if (w == 0) store_tile<T, NT, NRT>(acc, red, m, lane);
     __syncthreads();
@@ -507,17 +551,17 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
 }
 
 template <class T, int NT, int MAXRT>
-__global__ __launch_bounds__(64 * kWaves) void k_rows_mfma(
+__global__ __launch_bounds__(64 * kRowWaves) void k_rows_mfma(
     const RowTask *__restrict__ tasks, const int *__restrict__ leaf_start, const LeafPanel *__restrict__ lpanel,
     const int *__restrict__ psrc, const typename Mfma<T>::T2 *__restrict__ panel,
     const typename Mfma<T>::T2 *__restrict__ xt, const double *__restrict__ x, int N, int R, double *__restrict__ y,
     const int *__restrict__ cols, int j0, const int *__restrict__ gate, T *__restrict__ part) {
     // the reduction buffer, or (LDS-staged FP32 rows) the wavefronts' rings, whichever is larger
     constexpr bool lds_rows = kLdsRows && sizeof(T) == 4 && MAXRT == 2;
-    constexpr size_t ring_bytes = lds_rows ? (size_t)kWaves * LdsGeom<NT, 2>::WAVE : 0;
-    constexpr size_t bytes = std::max(sizeof(Lds<T, NT>), ring_bytes);
+    constexpr size_t ring_bytes = lds_rows ? (size_t)kRowWaves * LdsGeom<NT, 2>::WAVE : 0;
+    constexpr size_t bytes = std::max(sizeof(Lds<T, NT, kRowWaves>), ring_bytes);
     __shared__ __attribute__((aligned(16))) char smem[bytes];
-    Lds<T, NT> &red = *reinterpret_cast<Lds<T, NT> *>(smem);
+    Lds<T, NT, kRowWaves> &red = *reinterpret_cast<Lds<T, NT, kRowWaves> *>(smem);
     if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const RowTask T_ = tasks[blockIdx.x];
@@ -737,8 +781,8 @@ static void build_lists(arh_hmat *h, int tr) {
     // row tasks: leaves with more than tg groups are split evenly
     int64_t gt = 0;
     for (auto &L : M.lpanel) gt += L.ngroups;
-    int tg = (int)std::max<int64_t>(8 * kWaves, (gt + kTaskSlots - 1) / kTaskSlots);
-    tg = (tg + kWaves - 1) / kWaves * kWaves;
+    int tg = (int)std::max<int64_t>(8 * kRowWaves, (gt + kTaskSlots - 1) / kTaskSlots);
+    tg = (tg + kRowWaves - 1) / kRowWaves * kRowWaves;
     M.rtasks.clear();
     M.rsplit.clear();
     M.nslots = 0;
@@ -859,9 +903,9 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
     const int64_t e1 = M.vs_total;
     if (nu)
         timed(pi ? PROF_LR64 : PROF_LR32, eb * e1 + 16.0 * R * (N + h->t_total), 8.0 * R * e1, [&] {
-            hipLaunchKernelGGL((k_lr_mfma<T, NT>), dim3(nb + (nu - nb + kWaves - 1) / kWaves), dim3(64 * kWaves), 0,
-                               s, M.d_stacks.as<VStack>(), M.d_p1units.as<int2>(), nb, nu, M.vs[pi].as<T2>(), xt,
-                               gate);
+            hipLaunchKernelGGL((k_lr_mfma<T, NT>), dim3(nb + (nu - nb + kLrWaves - 1) / kLrWaves),
+                               dim3(64 * kLrWaves), 0, s, M.d_stacks.as<VStack>(), M.d_p1units.as<int2>(), nb, nu,
+                               M.vs[pi].as<T2>(), xt, gate);
         });
     const int64_t ent = g->dense_entries + (h->u_total + h->v_total - e1);
     T *part = M.nslots ? M.part[pi].as<T>() : nullptr;
@@ -870,12 +914,14 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
     double *yd = (double *)y;
     timed(pi ? PROF_ROWS64 : PROF_ROWS32, eb * ent + 16.0 * R * (2.0 * N + h->t_total), 8.0 * R * ent, [&] {
         if (maxrt <= 2) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 2>), dim3(ntask), dim3(64 * kWaves), 0, s, M.d_rtasks.as<RowTask>(),
-                               g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),
+            hipLaunchKernelGGL((k_rows_mfma<T, NT, 2>), dim3(ntask), dim3(64 * kRowWaves), 0, s,
+                               M.d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(),
+                               M.d_psrc.as<int>(),
                                M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         } else if constexpr (NT <= max_nt_tall<T>()) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 4>), dim3(ntask), dim3(64 * kWaves), 0, s, M.d_rtasks.as<RowTask>(),
-                               g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),
+            hipLaunchKernelGGL((k_rows_mfma<T, NT, 4>), dim3(ntask), dim3(64 * kRowWaves), 0, s,
+                               M.d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(),
+                               M.d_psrc.as<int>(),
                                M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
         } else {
             ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 32 elements");
