@@ -582,10 +582,12 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_mfma(
 }
 
 // y = -x/2 + the task partials of each split leaf, summed in task order. Workgroup (leaf, chunk) takes
-// kSumRows rows of one split leaf: every thread holds its few elements in registers and issues their
-// loads of one partial together, so a workgroup waits once per partial rather than once per element;
-// the summed rows go through LDS so that y rows are written contiguously.
+// kSumRows rows of one split leaf: every thread holds its few elements in registers and issues the loads
+// of up to kSumBatch partials together (indices clamped, surplus values not added), so a workgroup waits
+// about once per batch rather than once per partial; the summed rows go through LDS so that y rows are
+// written contiguously.
 constexpr int kSumRows = 8;
+constexpr int kSumBatch = 8;
 template <class T, int NT>
 __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split, const int *__restrict__ leaf_start,
                                                   const T *__restrict__ part, const double *__restrict__ x, int N,
@@ -603,13 +605,18 @@ __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split
     T acc[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) acc[i] = T(0);
-    for (int k = 0; k < S.z; k++) {
-        const T *sk = src + (int64_t)k * kPartRows * W;
+    for (int k0 = 0; k0 < S.z; k0 += kSumBatch) {
+        T v[kSumBatch][EPT];
 #pragma unroll
-        for (int i = 0; i < EPT; i++) {
-            const int e = (int)threadIdx.x + i * 256;
-            if (e < ne) acc[i] += sk[e];
+        for (int b = 0; b < kSumBatch; b++) {
+            const T *sk = src + (int64_t)min(k0 + b, S.z - 1) * kPartRows * W;
+#pragma unroll
+            for (int i = 0; i < EPT; i++) v[b][i] = sk[min((int)threadIdx.x + i * 256, ne - 1)];
         }
+#pragma unroll
+        for (int b = 0; b < kSumBatch; b++)
+#pragma unroll
+            for (int i = 0; i < EPT; i++) acc[i] = k0 + b < S.z ? acc[i] + v[b][i] : acc[i];
     }
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
