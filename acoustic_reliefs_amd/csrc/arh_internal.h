@@ -317,7 +317,7 @@ struct arh_qaca_s {
     arh_geom *g = nullptr;
     double k = 0, tol = 0;
     int LL = 0, N = 0, lim = 0, rank = 0;
-    int nbr = 0, nbc = 0;  // blocks over a row (N) and over a column (LL)
+    int nbr = 0, nbn = 0, nbc = 0;  // residual row blocks (N), row normalisation blocks (N), column blocks (LL)
     arh::DBuf L, U, V, used, part, dots, st, t;
 };
 

@@ -6,7 +6,9 @@
 // cross is a grid-wide kernel: residual row (N entries) with block-partial first-max, pivot, row
 // normalisation, residual column (LL entries) with masked first-max, the non-conjugated dot products
 // with the stored crosses (chunk partials over a 2-D grid), and a one-workgroup finish that applies
-// the stopping rule. All reductions are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
+// the stopping rule. A residual row or column block takes 64 entries (one per lane) and splits the sum
+// over the stored crosses between its kQS wavefronts, so the grid grows with the rank as well as with
+// N or LL. All reductions are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
 // checks), so the host launches a few crosses at a time and synchronises once per batch.
 //
 // Semantics kept from aca.cpp:84-134: start at row 0 (used = {0}); V row = v / v_j with no zero-pivot
@@ -21,6 +23,7 @@
 namespace arh {
 
 constexpr int kQT = 256;  // threads per block
+constexpr int kQS = 8;    // wavefronts of a residual row / column block (slices of the stored crosses)
 
 struct QAcaState {
     double2 piv;
@@ -55,33 +58,59 @@ __device__ __forceinline__ double block_sum_q(double v, double *sv) {
     return s;
 }
 
-// residual row i into V row `it` (unnormalised) + block partial first-max of |v|^2
-__global__ __launch_bounds__(kQT) void k_qa_row(const QAcaState *__restrict__ S, const double *__restrict__ L,
-                                               const Tri *__restrict__ tri, double k, int LL, int N,
-                                               const double2 *__restrict__ U, double2 *__restrict__ V,
-                                               double *__restrict__ pmax) {
-    __shared__ double sv[kQT / ARH_WAVE];
-    __shared__ int si[kQT / ARH_WAVE];
+// Slice w of the stored-cross sum of a residual entry: sum over p in wave w's share of [0, it) of
+// a(p) b(p), where the lane's own operand is own[p * ostride] and the shared one is shared[p * sstride].
+__device__ __forceinline__ double2 cross_slice(int it, int w, const double2 *__restrict__ own, int64_t ostride,
+                                               const double2 *__restrict__ shared, int64_t sstride) {
+    const int chunk = (it + kQS - 1) / kQS, p0 = w * chunk, p1 = min(it, p0 + chunk);
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 4
+    for (int p = p0; p < p1; p++) {
+        const double2 a = shared[p * sstride], b = own[p * ostride];
+        acc.x += a.x * b.x - a.y * b.y;
+        acc.y += a.x * b.y + a.y * b.x;
+    }
+    return acc;
+}
+
+// entry - (slice 0 + ... ): wave 0 subtracts the kQS slice sums in wave order (the other waves leave
+// theirs in red and return false)
+__device__ __forceinline__ bool combine_slices(double2 &v, const double2 &mine, double2 (*red)[ARH_WAVE], int w,
+                                               int lane) {
+    if (w > 0) red[w - 1][lane] = mine;
+    __syncthreads();
+    if (w != 0) return false;
+    v.x -= mine.x;
+    v.y -= mine.y;
+#pragma unroll
+    for (int q = 0; q < kQS - 1; q++) v.x -= red[q][lane].x, v.y -= red[q][lane].y;
+    return true;
+}
+
+// residual row i into V row `it` (unnormalised) + block partial first-max of |v|^2 (64 columns a block)
+__global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_row(const QAcaState *__restrict__ S,
+                                                          const double *__restrict__ L, const Tri *__restrict__ tri,
+                                                          double k, int LL, int N, const double2 *__restrict__ U,
+                                                          double2 *__restrict__ V, double *__restrict__ pmax) {
+    __shared__ double2 red[kQS - 1][ARH_WAVE];
     const QAcaState st = *S;
     if (st.done) return;
     const int i = st.i, it = st.it;
-    const double px = L[3 * i], py = L[3 * i + 1], pz = L[3 * i + 2];
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / ARH_WAVE)), lane = threadIdx.x % ARH_WAVE;
+    const int c = blockIdx.x * ARH_WAVE + lane, cc = min(c, N - 1);  // loads clamped, stores guarded
+    double2 v = make_double2(0.0, 0.0);
+    if (w == 0) v = bem_entry(k, L[3 * i], L[3 * i + 1], L[3 * i + 2], tri[cc]);
+    const double2 mine = cross_slice(it, w, V + cc, N, U + i, LL);
+    if (!combine_slices(v, mine, red, w, lane)) return;
     double best = -1.0;
     int bi = 0x7fffffff;
-    const int c = blockIdx.x * kQT + threadIdx.x;
     if (c < N) {
-        double2 v = bem_entry(k, px, py, pz, tri[c]);
-        for (int p = 0; p < it; p++) {
-            const double2 up = U[(int64_t)p * LL + i], vp = V[(int64_t)p * N + c];
-            v.x -= up.x * vp.x - up.y * vp.y;
-            v.y -= up.x * vp.y + up.y * vp.x;
-        }
         V[(int64_t)it * N + c] = v;
         const double a = cnorm2(v);
         best = a == a ? a : -1.0, bi = c;  // NaN never wins
     }
-    block_argmax_q(best, bi, sv, si);
-    if (threadIdx.x == 0) pmax[2 * blockIdx.x] = best, pmax[2 * blockIdx.x + 1] = (double)bi;
+    wave_argmax(best, bi);
+    if (lane == 0) pmax[2 * blockIdx.x] = best, pmax[2 * blockIdx.x + 1] = (double)bi;
 }
 
 // pivot column j = first max over the block partials (one wavefront; ties to the lower index, so
@@ -120,34 +149,34 @@ __global__ __launch_bounds__(kQT) void k_qa_norm(const QAcaState *__restrict__ S
     if (threadIdx.x == 0) pvn[blockIdx.x] = a;
 }
 
-// residual column j into U column `it` + block partials of |u|^2 and of the masked first-max
-__global__ __launch_bounds__(kQT) void k_qa_col(const QAcaState *__restrict__ S, const double *__restrict__ L,
-                                               const Tri *__restrict__ tri, double k, int LL, int N,
-                                               double2 *__restrict__ U, const double2 *__restrict__ V,
-                                               const unsigned char *__restrict__ used, double *__restrict__ pcol) {
-    __shared__ double sv[kQT / ARH_WAVE];
-    __shared__ int si[kQT / ARH_WAVE];
+// residual column j into U column `it` + block partials of |u|^2 and of the masked first-max (64 rows a
+// block)
+__global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_col(const QAcaState *__restrict__ S,
+                                                          const double *__restrict__ L, const Tri *__restrict__ tri,
+                                                          double k, int LL, int N, double2 *__restrict__ U,
+                                                          const double2 *__restrict__ V,
+                                                          const unsigned char *__restrict__ used,
+                                                          double *__restrict__ pcol) {
+    __shared__ double2 red[kQS - 1][ARH_WAVE];
     const QAcaState st = *S;
     if (st.done) return;
     const int j = st.j, it = st.it;
-    const Tri tj = tri[j];
-    const int l = blockIdx.x * kQT + threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / ARH_WAVE)), lane = threadIdx.x % ARH_WAVE;
+    const int l = blockIdx.x * ARH_WAVE + lane, ll = min(l, LL - 1);
+    double2 u = make_double2(0.0, 0.0);
+    if (w == 0) u = bem_entry(k, L[3 * ll], L[3 * ll + 1], L[3 * ll + 2], tri[j]);
+    const double2 mine = cross_slice(it, w, U + ll, LL, V + j, N);
+    if (!combine_slices(u, mine, red, w, lane)) return;
     double a = 0.0, best = -1.0;
     int bi = 0x7fffffff;
     if (l < LL) {
-        double2 u = bem_entry(k, L[3 * l], L[3 * l + 1], L[3 * l + 2], tj);
-        for (int p = 0; p < it; p++) {
-            const double2 up = U[(int64_t)p * LL + l], vp = V[(int64_t)p * N + j];
-            u.x -= up.x * vp.x - up.y * vp.y;
-            u.y -= up.x * vp.y + up.y * vp.x;
-        }
         U[(int64_t)it * LL + l] = u;
         a = cnorm2(u);
         if (!used[l]) best = a == a ? a : -1.0, bi = l;
     }
-    a = block_sum_q(a, sv);
-    block_argmax_q(best, bi, sv, si);
-    if (threadIdx.x == 0) pcol[3 * blockIdx.x] = a, pcol[3 * blockIdx.x + 1] = best, pcol[3 * blockIdx.x + 2] = (double)bi;
+    a = wave_sum(a);
+    wave_argmax(best, bi);
+    if (lane == 0) pcol[3 * blockIdx.x] = a, pcol[3 * blockIdx.x + 1] = best, pcol[3 * blockIdx.x + 2] = (double)bi;
 }
 
 // Chunk partials of the non-conjugated products u^T u_p and v v_p^T with the stored crosses p < it
@@ -180,7 +209,7 @@ __global__ __launch_bounds__(kQT) void k_qa_dots(const QAcaState *__restrict__ S
 
 // error estimate, stopping rule, next row (one workgroup; fixed-order reductions)
 __global__ __launch_bounds__(kQT) void k_qa_finish(QAcaState *__restrict__ S, double eps,
-                                                  const double *__restrict__ pvn, int nbr,
+                                                  const double *__restrict__ pvn, int nbn,
                                                   const double *__restrict__ pcol, int nbc,
                                                   const double2 *__restrict__ pu, int ncu,
                                                   const double2 *__restrict__ pv, int ncv,
@@ -191,7 +220,7 @@ __global__ __launch_bounds__(kQT) void k_qa_finish(QAcaState *__restrict__ S, do
     if (st0.done) return;
     double vn = 0.0, un = 0.0, d = 0.0, bv = -1.0;
     int bi = 0x7fffffff;
-    for (int b = threadIdx.x; b < nbr; b += kQT) vn += pvn[b];
+    for (int b = threadIdx.x; b < nbn; b += kQT) vn += pvn[b];
     for (int b = threadIdx.x; b < nbc; b += kQT) {
         un += pcol[3 * b];
         const double v = pcol[3 * b + 1];
@@ -291,9 +320,10 @@ void qaca_build(arh_qaca *q, const double *L_host) {
     upload(q->L, hL, s);
     q->U.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)LL * lim));
     q->V.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)lim * N));
-    q->nbr = (N + kQT - 1) / kQT;
-    q->nbc = (LL + kQT - 1) / kQT;
-    q->part.alloc(sizeof(double) * (2 * (size_t)q->nbr + q->nbr + 3 * (size_t)q->nbc + 16));
+    q->nbr = (N + ARH_WAVE - 1) / ARH_WAVE;  // residual row blocks
+    q->nbn = (N + kQT - 1) / kQT;            // row normalisation blocks
+    q->nbc = (LL + ARH_WAVE - 1) / ARH_WAVE;  // residual column blocks
+    q->part.alloc(sizeof(double) * (2 * (size_t)q->nbr + q->nbn + 3 * (size_t)q->nbc + 16));
     const int ncu = (LL + kQC - 1) / kQC, ncv = (N + kQC - 1) / kQC;
     q->dots.alloc(sizeof(double2) * (size_t)std::max(1, lim) * (ncu + ncv));
     std::vector<unsigned char> used(LL, 0);
@@ -304,7 +334,7 @@ void qaca_build(arh_qaca *q, const double *L_host) {
     st0.i = 0, st0.j = 0, st0.it = 0, st0.done = lim <= 0 ? 1 : 0, st0.lim = lim;
     q->st.alloc(sizeof(QAcaState));
     ARH_HIP(hipMemcpyAsync(q->st.p, &st0, sizeof(QAcaState), hipMemcpyHostToDevice, s));
-    double *pmax = q->part.as<double>(), *pvn = pmax + 2 * q->nbr, *pcol = pvn + q->nbr;
+    double *pmax = q->part.as<double>(), *pvn = pmax + 2 * q->nbr, *pcol = pvn + q->nbn;
     QAcaState *S = q->st.as<QAcaState>();
     const Tri *tri = g->tri.as<Tri>();
     const double2 *Uc = q->U.as<double2>();
@@ -318,16 +348,16 @@ void qaca_build(arh_qaca *q, const double *L_host) {
         const int batch = std::min(std::max(8, std::min(32, launched / 4)), lim - launched);
         for (int b = 0; b < batch; b++) {
             const int itb = launched + b;  // the cross index if none of the previous ones stopped
-            hipLaunchKernelGGL(k_qa_row, dim3(q->nbr), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N, Uc,
-                               q->V.as<double2>(), pmax);
+            hipLaunchKernelGGL(k_qa_row, dim3(q->nbr), dim3(kQS * ARH_WAVE), 0, s, S, q->L.as<double>(), tri, q->k,
+                               LL, N, Uc, q->V.as<double2>(), pmax);
             hipLaunchKernelGGL(k_qa_pivot, dim3(1), dim3(ARH_WAVE), 0, s, S, pmax, q->nbr, N, Vc);
-            hipLaunchKernelGGL(k_qa_norm, dim3(q->nbr), dim3(kQT), 0, s, S, N, q->V.as<double2>(), pvn);
-            hipLaunchKernelGGL(k_qa_col, dim3(q->nbc), dim3(kQT), 0, s, S, q->L.as<double>(), tri, q->k, LL, N,
-                               q->U.as<double2>(), Vc, q->used.as<unsigned char>(), pcol);
+            hipLaunchKernelGGL(k_qa_norm, dim3(q->nbn), dim3(kQT), 0, s, S, N, q->V.as<double2>(), pvn);
+            hipLaunchKernelGGL(k_qa_col, dim3(q->nbc), dim3(kQS * ARH_WAVE), 0, s, S, q->L.as<double>(), tri, q->k,
+                               LL, N, q->U.as<double2>(), Vc, q->used.as<unsigned char>(), pcol);
             if (itb > 0)
                 hipLaunchKernelGGL(k_qa_dots, dim3(itb, ncu + ncv), dim3(kQT), 0, s, S, LL, N, ncu, ncv, Uc, Vc, pu,
                                    pv);
-            hipLaunchKernelGGL(k_qa_finish, dim3(1), dim3(kQT), 0, s, S, q->tol, pvn, q->nbr, pcol, q->nbc, pu, ncu,
+            hipLaunchKernelGGL(k_qa_finish, dim3(1), dim3(kQT), 0, s, S, q->tol, pvn, q->nbn, pcol, q->nbc, pu, ncu,
                                pv, ncv, q->used.as<unsigned char>());
         }
         ARH_HIP(hipGetLastError());
