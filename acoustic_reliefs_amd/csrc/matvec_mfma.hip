@@ -107,7 +107,10 @@ constexpr int kRowWaves = ARH_MV_RWAVES;  // row tasks
 constexpr bool kChunked = ARH_MV_CHUNKED;  // a workgroup's waves take contiguous group ranges (else strided)
 constexpr bool kXcdOrder = ARH_MV_XCD;     // row tasks ordered for the per-die L2s
 constexpr int kXcds = 8;                   // dies (XCDs) of an MI355X, each with its own L2
-constexpr int kBigCols = 256;     // stacks with at least this many columns get a whole workgroup per unit
+#ifndef ARH_MV_BIGCOLS
+#define ARH_MV_BIGCOLS 256
+#endif
+constexpr int kBigCols = ARH_MV_BIGCOLS;  // stacks with at least this many columns get a whole workgroup per unit
 constexpr int kPartRows = 64;     // rows of a partial tile (the largest leaf)
 constexpr int kTaskSlots = ARH_MV_SLOTS;  // row tasks aimed for: a few per compute unit
 // FP32 row tasks of leaves <= 32 rows stage operands in LDS (global_load_lds). Off: bit for bit the
