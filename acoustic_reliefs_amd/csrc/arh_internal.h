@@ -292,7 +292,8 @@ struct arh_hmat_s {
     int storage = ARH_FP64;
     int max_rank = 0;
     double aca_tol = 0;
-    arh::DBuf dense, dense32;  // panels (c128 / c64)
+    arh::DBuf dense, dense32;    // panels (c128 / c64), column-major per leaf
+    arh::DBuf denseT, denseT32;  // the same panels row-major per leaf (generic transposed product, built on use)
     arh::DBuf blocks;          // LRBlock[na]
     arh::DBuf U, V, U32, V32;  // packed factors
     std::vector<LRBlock> hblocks;

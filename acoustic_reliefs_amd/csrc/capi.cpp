@@ -294,8 +294,8 @@ static void fill_info(arh_hmat *h) {
     I.sum_rank = 0;
     for (auto &B : h->hblocks) I.max_rank = std::max(I.max_rank, B.rank_ref), I.sum_rank += B.rank;
     I.bytes_fp64 = (int64_t)sizeof(double2) * (g->dense_entries + h->u_total + h->v_total);
-    I.bytes_fp32 = (int64_t)(h->dense32.bytes + h->U32.bytes + h->V32.bytes);
-    int64_t fp64_extra = 0;
+    I.bytes_fp32 = (int64_t)(h->dense32.bytes + h->denseT32.bytes + h->U32.bytes + h->V32.bytes);
+    int64_t fp64_extra = (int64_t)h->denseT.bytes;
     for (auto &L : h->mf) I.bytes_fp32 += (int64_t)(L.panel[0].bytes + L.vs[0].bytes);
     for (auto &L : h->mf) fp64_extra += (int64_t)(L.panel[1].bytes + L.vs[1].bytes);
     I.bytes_total = (int64_t)(h->dense.bytes + h->U.bytes + h->V.bytes + h->blocks.bytes + h->d_active_lr.bytes) +
@@ -404,6 +404,7 @@ int arh_hmat_compact(arh_hmat *h) {
                 L.built[p] = false;
             }
         h->dense32.reset(), h->U32.reset(), h->V32.reset(), h->tbuf.reset();
+        h->denseT.reset(), h->denseT32.reset();
         h->generic32 = false;
         fill_info(h);
     });

@@ -10,6 +10,8 @@
 // Transposed (set_transposed, hmatrix.h:95-115):
 //   1. per low-rank block b:   s_b = U_b^T x[rs:re]
 //   2. per column leaf l:      y[l] = -x[l]/2 + sum_{dense (l', l)} D^T x[l'] + sum_{b covers l} V_b[:, l]^T s_b
+//      Lanes take the columns of D, so D is read from a row-major copy of the panels (a row of a block is
+//      then one contiguous run); the copy is made on the first transposed product.
 #include <algorithm>
 
 #include "arh_internal.h"
@@ -170,23 +172,43 @@ __global__ __launch_bounds__(kMvThreads) void k_rows(const int *__restrict__ lea
     }
 }
 
-// ---------------------------------------------------------------- phase 2 (transposed): one workgroup per column leaf
+// Row-major copy of each leaf's panel: out[off + i ncol + c] = in[off + c m + i] (one workgroup per leaf).
 template <class S>
-__global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ leaf_start, const int *__restrict__ dT_ptr,
+__global__ __launch_bounds__(kMvThreads) void k_panel_rowmajor(const int *__restrict__ leaf_start,
+                                                               const int *__restrict__ pcol_ptr,
+                                                               const int64_t *__restrict__ panel_off,
+                                                               const S *__restrict__ in, S *__restrict__ out) {
+    const int l = blockIdx.x;
+    const int m = leaf_start[l + 1] - leaf_start[l], ncol = pcol_ptr[l + 1] - pcol_ptr[l];
+    const int64_t off = panel_off[l], tot = (int64_t)m * ncol;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int i = (int)(e / ncol), c = (int)(e - (int64_t)i * ncol);
+        out[off + e] = in[off + (int64_t)c * m + i];
+    }
+}
+
+// ---------------------------------------------------------------- phase 2 (transposed): workgroup (column leaf, 64 columns)
+// W wavefronts take turns over the leaf's dense blocks and over the ranks of its low-rank blocks.
+constexpr int kColWaves = 16;
+template <class S, int W>
+__global__ __launch_bounds__(64 * W) void k_cols(const int *__restrict__ leaf_start, const int *__restrict__ dT_ptr,
                                                      const int *__restrict__ dT_blk, const int *__restrict__ dblk_info,
-                                                     const int64_t *__restrict__ panel_off, const S *__restrict__ panel,
+                                                     const int *__restrict__ pcol_ptr,
+                                                     const int64_t *__restrict__ panel_off, const S *__restrict__ panelT,
                                                      const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
                                                      const LRBlock *__restrict__ blocks, const S *__restrict__ V,
                                                      int vcm, const double2 *__restrict__ t,
                                                      const double2 *__restrict__ x, int N, int R,
                                                      double2 *__restrict__ y) {
-    __shared__ double2 red[kMvWaves * ARH_WAVE * RC];
+    __shared__ double2 red[W * ARH_WAVE * RC];
     const int l = blockIdx.x;
     const int cs = leaf_start[l], n = leaf_start[l + 1] - cs;
     const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
     const int d0 = dT_ptr[l], d1 = dT_ptr[l + 1];
     const int b0 = lr_ptr[l], b1 = lr_ptr[l + 1];
-    for (int c0 = 0; c0 < n; c0 += ARH_WAVE) {
+    {
+        const int c0 = (int)blockIdx.y * ARH_WAVE;
+        if (c0 >= n) return;
         const int c = c0 + lane;
         const bool on = c < n;
         for (int j0 = 0; j0 < R; j0 += RC) {
@@ -194,13 +216,14 @@ __global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ lea
 #pragma unroll
             for (int q = 0; q < RC; q++) acc[q] = make_double2(0.0, 0.0);
             if (on) {
-                for (int e = d0 + w; e < d1; e += kMvWaves) {
+                for (int e = d0 + w; e < d1; e += W) {
                     const int b = dT_blk[e];
                     const int lr = dblk_info[2 * b], pc = dblk_info[2 * b + 1];
                     const int rs = leaf_start[lr], m = leaf_start[lr + 1] - rs;
-                    const S *D = panel + panel_off[lr] + (int64_t)pc * m + (int64_t)c * m;
+                    const int64_t ncol = pcol_ptr[lr + 1] - pcol_ptr[lr];
+                    const S *D = panelT + panel_off[lr] + pc + c;  // row i of the block at D + i ncol
                     for (int i = 0; i < m; i++) {
-                        double2 a = CT<S>::ld(D + i);
+                        double2 a = CT<S>::ld(D + i * ncol);
 #pragma unroll
                         for (int q = 0; q < RC; q++)
                             if (j0 + q < R) cfma(acc[q], a, x[(int64_t)(j0 + q) * N + rs + i]);
@@ -209,7 +232,7 @@ __global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ lea
                 for (int e = b0; e < b1; e++) {
                     const LRBlock &B = blocks[lr_blk[e]];
                     const int r = B.rank;
-                    for (int p = w; p < r; p += kMvWaves) {
+                    for (int p = w; p < r; p += W) {
                         double2 a = CT<S>::ld(v_at(V, B, p, (cs - B.cs) + c, vcm));
 #pragma unroll
                         for (int q = 0; q < RC; q++)
@@ -227,7 +250,7 @@ __global__ __launch_bounds__(kMvThreads) void k_cols(const int *__restrict__ lea
                 if (j0 + q < R) {
                     const int64_t yo = (int64_t)(j0 + q) * N + cs + c0 + cc;
                     double2 s = make_double2(-0.5 * x[yo].x, -0.5 * x[yo].y);
-                    for (int ww = 0; ww < kMvWaves; ww++) {
+                    for (int ww = 0; ww < W; ww++) {
                         double2 v = red[(ww * ARH_WAVE + cc) * RC + q];
                         s.x += v.x, s.y += v.y;
                     }
@@ -279,12 +302,27 @@ static void apply_t(arh_hmat *h, int transpose, int R, const S *panel, const S *
                                g->d_lrR_ptr.as<int>(), g->d_lrR_blk.as<int>(), h->blocks.as<LRBlock>(), U, t, x, N, R,
                                y);
         else
-            hipLaunchKernelGGL(k_cols<S>, dim3(g->nleaf), dim3(kMvThreads), 0, s, g->d_leaf_start.as<int>(),
-                               g->d_dT_ptr.as<int>(), g->d_dT_blk.as<int>(), g->d_dblk_info.as<int>(),
-                               g->d_panel_off.as<int64_t>(), panel, g->d_lrC_ptr.as<int>(), g->d_lrC_blk.as<int>(),
-                               h->blocks.as<LRBlock>(), V, (int)fp32, t, x, N, R, y);
+            hipLaunchKernelGGL((k_cols<S, kColWaves>), dim3(g->nleaf, (g->max_leaf + ARH_WAVE - 1) / ARH_WAVE),
+                               dim3(64 * kColWaves), 0, s,
+                               g->d_leaf_start.as<int>(), g->d_dT_ptr.as<int>(), g->d_dT_blk.as<int>(),
+                               g->d_dblk_info.as<int>(), g->d_pcol_ptr.as<int>(), g->d_panel_off.as<int64_t>(), panel,
+                               g->d_lrC_ptr.as<int>(), g->d_lrC_blk.as<int>(), h->blocks.as<LRBlock>(), V, (int)fp32,
+                               t, x, N, R, y);
     });
     ARH_HIP(hipGetLastError());
+}
+
+// The row-major panel copy for the transposed product (made once per H-matrix and precision).
+template <class S>
+static const S *rowmajor_panels(arh_hmat *h, DBuf &src, DBuf &dst, hipStream_t s) {
+    arh_geom *g = h->g;
+    if (!dst.p) {
+        dst.alloc(sizeof(S) * std::max<int64_t>(1, g->dense_entries));
+        hipLaunchKernelGGL(k_panel_rowmajor<S>, dim3(g->nleaf), dim3(kMvThreads), 0, s, g->d_leaf_start.as<int>(),
+                           g->d_pcol_ptr.as<int>(), g->d_panel_off.as<int64_t>(), src.as<S>(), dst.as<S>());
+        ARH_HIP(hipGetLastError());
+    }
+    return dst.as<S>();
 }
 
 void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const double2 *x, double2 *y, hipStream_t s,
@@ -299,7 +337,9 @@ void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const doubl
             return;
         }
         make_generic_fp32(h);
-        apply_t<float2>(h, transpose, nrhs, h->dense32.as<float2>(), h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
+        const float2 *panel = h->dense32.as<float2>();
+        if (transpose) panel = rowmajor_panels<float2>(h, h->dense32, h->denseT32, s);
+        apply_t<float2>(h, transpose, nrhs, panel, h->U32.as<float2>(), h->V32.as<float2>(), x, y, s);
     } else {
         if (h->g->max_leaf <= mfma_max_leaf()) {
             hmat_apply_mfma(h, transpose, 1, nrhs, x, y, s, cols, ncols, gate);
@@ -309,7 +349,8 @@ void hmat_apply(arh_hmat *h, int transpose, int precision, int nrhs, const doubl
             hmat_apply64(h, nrhs, x, y, s);
             return;
         }
-        apply_t<double2>(h, transpose, nrhs, h->dense.as<double2>(), h->U.as<double2>(), h->V.as<double2>(), x, y, s);
+        apply_t<double2>(h, transpose, nrhs, rowmajor_panels<double2>(h, h->dense, h->denseT, s), h->U.as<double2>(),
+                         h->V.as<double2>(), x, y, s);
     }
 }
 
