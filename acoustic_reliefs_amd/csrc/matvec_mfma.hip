@@ -111,7 +111,8 @@ constexpr int kXcds = 8;                   // dies (XCDs) of an MI355X, each wit
 #define ARH_MV_BIGCOLS 256
 #endif
 constexpr int kBigCols = ARH_MV_BIGCOLS;  // stacks with at least this many columns get a whole workgroup per unit
-constexpr int kPartRows = 64;     // rows of a partial tile (the largest leaf)
+constexpr int kPartRows = 64;     // rows of a partial tile for leaves of up to 64 elements
+constexpr int kMaxLeaf = 128;     // the largest leaf the engine takes (8 row tiles; partial tiles of 128 rows)
 constexpr int kTaskSlots = ARH_MV_SLOTS;  // row tasks aimed for: a few per compute unit
 // FP32 row tasks of leaves <= 32 rows stage operands in LDS (global_load_lds). Off: bit for bit the
 // same results, but 126 us against 93 us for the register path on the 4000 Hz / 37 rhs product (one
@@ -300,10 +301,10 @@ __device__ __forceinline__ int3 wave_share(int n, int w) {
 }
 
 // LDS of a reduction / output staging: W / 2 slots of NT * 256 values (one row tile each), and room for
-// kPartRows x 16 NT values of an output tile.
-template <class T, int NT, int W>
+// PR x 16 NT values of an output tile.
+template <class T, int NT, int W, int PR = kPartRows>
 struct Lds {
-    T v[std::max((W / 2) * NT * 256, kPartRows * 16 * NT)];
+    T v[std::max((W / 2) * NT * 256, PR * 16 * NT)];
 };
 
 // Fixed-order tree reduction of the W wave partials into wave 0, one row tile at a time:
@@ -516,7 +517,7 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
                                          const typename Mfma<T>::T2 *__restrict__ panel,
                                          const typename Mfma<T>::T2 *__restrict__ xt, const double *__restrict__ x,
                                          int N, int R, double *__restrict__ y, const int *__restrict__ cols, int j0,
-                                         T *__restrict__ part, T *red, int w, int lane) {
+                                         T *__restrict__ part, int prow, T *red, int w, int lane) {
     typedef typename Mfma<T>::T2 T2;
     constexpr int RP = 8 * NT;
     const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
@@ -542,7 +543,7 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
     if (w == 0) store_tile<T, NT, NRT>(acc, red, m, lane);
     __syncthreads();
     if (T_.slot >= 0) {
-        T *dst = part + (int64_t)T_.slot * kPartRows * (2 * RP);
+        T *dst = part + (int64_t)T_.slot * prow * (2 * RP);
         for (int e = threadIdx.x; e < m * 2 * RP; e += blockDim.x) dst[e] = red[e];
     } else {
         for (int e = threadIdx.x; e < 2 * m * R; e += blockDim.x) {
@@ -558,13 +559,14 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_mfma(
     const RowTask *__restrict__ tasks, const int *__restrict__ leaf_start, const LeafPanel *__restrict__ lpanel,
     const int *__restrict__ psrc, const typename Mfma<T>::T2 *__restrict__ panel,
     const typename Mfma<T>::T2 *__restrict__ xt, const double *__restrict__ x, int N, int R, double *__restrict__ y,
-    const int *__restrict__ cols, int j0, const int *__restrict__ gate, T *__restrict__ part) {
+    const int *__restrict__ cols, int j0, const int *__restrict__ gate, T *__restrict__ part, int prow) {
     // the reduction buffer, or (LDS-staged FP32 rows) the wavefronts' rings, whichever is larger
+    constexpr int PR = MAXRT > 4 ? kMaxLeaf : kPartRows;  // output staging rows
     constexpr bool lds_rows = kLdsRows && sizeof(T) == 4 && MAXRT == 2;
     constexpr size_t ring_bytes = lds_rows ? (size_t)kRowWaves * LdsGeom<NT, 2>::WAVE : 0;
-    constexpr size_t bytes = std::max(sizeof(Lds<T, NT, kRowWaves>), ring_bytes);
+    constexpr size_t bytes = std::max(sizeof(Lds<T, NT, kRowWaves, PR>), ring_bytes);
     __shared__ __attribute__((aligned(16))) char smem[bytes];
-    Lds<T, NT, kRowWaves> &red = *reinterpret_cast<Lds<T, NT, kRowWaves> *>(smem);
+    Lds<T, NT, kRowWaves, PR> &red = *reinterpret_cast<Lds<T, NT, kRowWaves, PR> *>(smem);
     if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const RowTask T_ = tasks[blockIdx.x];
@@ -573,14 +575,21 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_mfma(
     const LeafPanel L = lpanel[T_.leaf];
     const int nrt = (m + 15) >> 4;
     if (nrt <= 1)
-        rows_run<T, NT, 1>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red.v, w, lane);
+        rows_run<T, NT, 1>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
     else if (nrt == 2 || MAXRT == 2)
-        rows_run<T, NT, 2>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red.v, w, lane);
-    else if constexpr (MAXRT >= 4) {
+        rows_run<T, NT, 2>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
+    else if constexpr (MAXRT == 4) {
         if (nrt == 3)
-            rows_run<T, NT, 3>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red.v, w, lane);
+            rows_run<T, NT, 3>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
         else
-            rows_run<T, NT, 4>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, red.v, w, lane);
+            rows_run<T, NT, 4>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
+    } else if constexpr (MAXRT == 8) {
+        if (nrt <= 4)
+            rows_run<T, NT, 4>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
+        else if (nrt <= 6)
+            rows_run<T, NT, 6>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
+        else
+            rows_run<T, NT, 8>(T_, rs, m, L, psrc, panel, xt, x, N, R, y, cols, j0, part, prow, red.v, w, lane);
     }
 }
 
@@ -595,7 +604,7 @@ template <class T, int NT>
 __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split, const int *__restrict__ leaf_start,
                                                   const T *__restrict__ part, const double *__restrict__ x, int N,
                                                   int R, double *__restrict__ y, const int *__restrict__ cols, int j0,
-                                                  const int *__restrict__ gate) {
+                                                  const int *__restrict__ gate, int prow) {
     constexpr int W = 16 * NT, E = kSumRows * W, EPT = (E + 255) / 256;
     __shared__ T tile[E];
     if (gated(gate)) return;
@@ -604,7 +613,7 @@ __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split
     const int r0 = (int)blockIdx.y * kSumRows;
     if (r0 >= m) return;
     const int mr = min(kSumRows, m - r0), ne = mr * W;
-    const T *src = part + (int64_t)S.y * kPartRows * W + (int64_t)r0 * W;
+    const T *src = part + (int64_t)S.y * prow * W + (int64_t)r0 * W;
     T acc[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) acc[i] = T(0);
@@ -612,7 +621,7 @@ __global__ __launch_bounds__(256) void k_rows_sum(const int4 *__restrict__ split
         T v[kSumBatch][EPT];
 #pragma unroll
         for (int b = 0; b < kSumBatch; b++) {
-            const T *sk = src + (int64_t)min(k0 + b, S.z - 1) * kPartRows * W;
+            const T *sk = src + (int64_t)min(k0 + b, S.z - 1) * prow * W;
 #pragma unroll
             for (int i = 0; i < EPT; i++) v[b][i] = sk[min((int)threadIdx.x + i * 256, ne - 1)];
         }
@@ -691,7 +700,7 @@ static void build_lists(arh_hmat *h, int tr) {
     MfmaLayout &M = h->mf[tr];
     hipStream_t s = g->ctx->stream;
     const int N = g->N;
-    ARH_CHECK(g->max_leaf <= kPartRows, ARH_EINVAL, "MFMA product supports leaves of at most 64 elements");
+    ARH_CHECK(g->max_leaf <= kMaxLeaf, ARH_EINVAL, "MFMA product supports leaves of at most 128 elements");
     // stacks: active low-rank blocks grouped by their column (forward) / row (transposed) cluster,
     // in order of first appearance
     auto cluster = [&](const LRBlock &B) { return tr ? std::make_pair(B.rs, B.re) : std::make_pair(B.cs, B.ce); };
@@ -876,11 +885,19 @@ void mfma_prepare(arh_hmat *h, int tr, int fp64) {
         build_copy<float>(h, tr, 0);
 }
 
-// Widest pass (NT output tiles) that fits the registers with 4 row tiles (leaves over 32 elements).
+// Widest pass (NT output tiles) that fits the registers with 4 row tiles (leaves over 32 elements) and with
+// 8 row tiles (leaves over 64 elements).
 template <class T>
 constexpr int max_nt_tall() {
     return sizeof(T) == 8 ? 3 : 5;
 }
+template <class T>
+constexpr int max_nt_xtall() {
+    return sizeof(T) == 8 ? 1 : 2;
+}
+
+// Rows of a partial tile: the largest leaf's row tiles (64 rows up to 4 tiles, else 128).
+inline int part_rows(int maxrt) { return maxrt > 4 ? kMaxLeaf : kPartRows; }
 
 template <class T, int NT>
 static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, double2 *y, const int *cols, int j0,
@@ -892,7 +909,8 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
     const int N = g->N;
     constexpr int RP = 8 * NT;
     M.xt[pi].alloc(sizeof(T2) * (size_t)(N + h->t_total) * RP);
-    if (M.nslots) M.part[pi].alloc(sizeof(T) * (size_t)M.nslots * kPartRows * 2 * RP);
+    const int prow = part_rows(maxrt);
+    if (M.nslots) M.part[pi].alloc(sizeof(T) * (size_t)M.nslots * prow * 2 * RP);
     T2 *xt = M.xt[pi].as<T2>();
     hipLaunchKernelGGL(k_pack_xf<T>, dim3((unsigned)std::min<int64_t>(2048, ((int64_t)N * RP + 255) / 256)), dim3(256),
                        0, s, x, N, R, RP, xt, cols, j0, gate);
@@ -923,27 +941,32 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
     const double *xd = (const double *)x;
     double *yd = (double *)y;
     timed(pi ? PROF_ROWS64 : PROF_ROWS32, eb * ent + 16.0 * R * (2.0 * N + h->t_total), 8.0 * R * ent, [&] {
+        auto launch_rows = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(ntask), dim3(64 * kRowWaves), 0, s, M.d_rtasks.as<RowTask>(),
+                               g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),
+                               M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part, prow);
+        };
         if (maxrt <= 2) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 2>), dim3(ntask), dim3(64 * kRowWaves), 0, s,
-                               M.d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(),
-                               M.d_psrc.as<int>(),
-                               M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
-        } else if constexpr (NT <= max_nt_tall<T>()) {
-            hipLaunchKernelGGL((k_rows_mfma<T, NT, 4>), dim3(ntask), dim3(64 * kRowWaves), 0, s,
-                               M.d_rtasks.as<RowTask>(), g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(),
-                               M.d_psrc.as<int>(),
-                               M.panel[pi].as<T2>(), (const T2 *)xt, xd, N, R, yd, cols, j0, gate, part);
+            launch_rows(k_rows_mfma<T, NT, 2>);
+        } else if (maxrt <= 4) {
+            if constexpr (NT <= max_nt_tall<T>())
+                launch_rows(k_rows_mfma<T, NT, 4>);
+            else
+                ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 32 elements");
         } else {
-            ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 32 elements");
+            if constexpr (NT <= max_nt_xtall<T>())
+                launch_rows(k_rows_mfma<T, NT, 8>);
+            else
+                ARH_CHECK(false, ARH_EINVAL, "internal: pass too wide for leaves of more than 64 elements");
         }
     });
     if (nsp)
         hipLaunchKernelGGL((k_rows_sum<T, NT>), dim3(nsp, (g->max_leaf + kSumRows - 1) / kSumRows), dim3(256), 0, s,
-                           M.d_rsplit.as<int4>(), g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate);
+                           M.d_rsplit.as<int4>(), g->d_leaf_start.as<int>(), part, xd, N, R, yd, cols, j0, gate, prow);
     ARH_HIP(hipGetLastError());
 }
 
-int mfma_max_leaf() { return kPartRows; }
+int mfma_max_leaf() { return kMaxLeaf; }
 
 // y = (M - I/2) x, or its transpose, on the MFMA layout of one precision, right-hand sides in passes
 // of up to kMaxRC. With a column list (device array of ncols indices into the nrhs columns of x
@@ -956,7 +979,7 @@ static void apply_t(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, double2
     const int maxrt = (g->max_leaf + 15) / 16;
     const int nc = cols ? ncols : R;
     ARH_CHECK(nc >= 0 && nc <= R, ARH_EINVAL, "column list longer than the number of right-hand sides");
-    const int maxrc = maxrt > 2 ? 8 * max_nt_tall<T>() : kMaxRC;
+    const int maxrc = maxrt > 4 ? 8 * max_nt_xtall<T>() : maxrt > 2 ? 8 * max_nt_tall<T>() : kMaxRC;
     for (int j0 = 0; j0 < nc; j0 += maxrc) {
         const int Rc = std::min(maxrc, nc - j0);
         switch ((Rc + 7) / 8) {
@@ -971,7 +994,7 @@ static void apply_t(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, double2
 
 void hmat_apply_mfma(arh_hmat *h, int transpose, int fp64, int R, const double2 *x, double2 *y, hipStream_t s,
                      const int *cols, int ncols, const int *gate) {
-    ARH_CHECK(h->g->max_leaf <= mfma_max_leaf(), ARH_EINVAL, "MFMA product supports leaves of at most 64 elements");
+    ARH_CHECK(h->g->max_leaf <= mfma_max_leaf(), ARH_EINVAL, "MFMA product supports leaves of at most 128 elements");
     const int tr = transpose ? 1 : 0;
     mfma_prepare(h, tr, fp64 ? 1 : 0);
     if (fp64)

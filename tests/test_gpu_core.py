@@ -124,15 +124,16 @@ def test_receivers_and_diffusion(sphere_setup, ctx, oracle):
 
 
 @pytest.mark.parametrize("storage", ["fp64", "mixed"])
-def test_matvec_large_leaves(ctx, capi, oracle, storage):
-    """Leaves of up to 128 elements (DiffBEM's cluster_size 128) take the generic products (row-owner
-    kernels, the transposed one reading the row-major panel copy), not the MFMA engine: forward and
-    transposed products against the oracle's and the dense matrix, FP64 and FP32 storage, several rhs
-    (icosphere of 5120 triangles: leaves up to 128 and a few hundred low-rank blocks)."""
+@pytest.mark.parametrize("leaf,lo,hi", [(128, 64, 128), (256, 128, 256)])
+def test_matvec_large_leaves(ctx, capi, oracle, storage, leaf, lo, hi):
+    """Leaves of 65-128 elements (DiffBEM's cluster_size 128) take the MFMA engine's 8-row-tile class; larger
+    ones the generic products (row-owner kernels, the transposed one reading the row-major panel copy).
+    Forward and transposed products against the oracle's and the dense matrix, FP64 and FP32 storage,
+    several rhs (icosphere of 5120 triangles)."""
     from acoustic_reliefs_amd import meshes
     P, E = meshes.icosphere(4, 0.5)
-    g = capi.Geometry.from_mesh(ctx, P, E, leaf=128, factor=1.5)
-    assert max(int(b[1] - b[0]) for b in g.dense) > 64
+    g = capi.Geometry.from_mesh(ctx, P, E, leaf=leaf, factor=1.5)
+    assert lo < max(int(b[1] - b[0]) for b in g.dense) <= hi
     oc = oracle.Ctx(P, g.E, g.dense, g.lowrank)
     oc.assemble(K500, 1e-4, 100)
     st = capi.ARH_FP64 if storage == "fp64" else capi.ARH_MIXED
@@ -142,13 +143,14 @@ def test_matvec_large_leaves(ctx, capi, oracle, storage):
     X = rng.standard_normal((3, g.N)) + 1j * rng.standard_normal((3, g.N))
     tol = 1e-4 if storage == "fp64" else 1e-4 + 5e-6
     for tr in (False, True):
-        print("large leaves %s transpose=%s" % (storage, tr), flush=True)
+        print("large leaves %d %s transpose=%s" % (leaf, storage, tr), flush=True)
         Y = h.matvec_np(X, transpose=tr, precision=st)
         for j in range(3):
             yo = oc.matvec(X[j], transpose=tr)
             yd = (A.T if tr else A) @ X[j]
             assert rel(Y[j], yo) < tol, (tr, j, rel(Y[j], yo))
             assert rel(Y[j], yd) < 1e-3
-        # a second transposed product reuses the row-major copy
-        if tr:
-            assert np.array_equal(h.matvec_np(X, transpose=tr, precision=st), Y)
+        # a second product gives the same bits (the transposed one reuses its layout copy)
+        assert np.array_equal(h.matvec_np(X, transpose=tr, precision=st), Y)
+        # one rhs alone gets the bits it gets in the batch
+        assert np.array_equal(h.matvec_np(X[1], transpose=tr, precision=st), Y[1])
