@@ -3,10 +3,10 @@
 // its transpose V^T (U^T x) (adjoint.cpp:101-102, 177-178).
 //
 // Unlike the block ACA (one workgroup per block), this is a single large block, so every stage of a
-// cross is a grid-wide kernel: residual row (N entries) with block-partial first-max, pivot, row
-// normalisation, residual column (LL entries) with masked first-max, the non-conjugated dot products
-// with the stored crosses (chunk partials over a 2-D grid), and a one-workgroup finish that applies
-// the stopping rule. A residual row or column block takes 64 entries (one per lane) and splits the sum
+// cross is a grid-wide kernel: residual row (N entries) with block-partial first-max; pivot, row
+// normalisation and residual column (LL entries, masked first-max) in one launch; the non-conjugated dot
+// products with the stored crosses (chunk partials over a 2-D grid); and a one-workgroup finish that
+// applies the stopping rule. A residual row or column block takes 64 entries (one per lane) and splits the sum
 // over the stored crosses between its kQS wavefronts, so the grid grows with the rank as well as with
 // N or LL. All reductions are two-stage with a fixed order. The loop control lives on the device (a `done` flag every kernel
 // checks), so the host launches a few crosses at a time and synchronises once per batch.
@@ -26,10 +26,8 @@ constexpr int kQT = 256;  // threads per block
 constexpr int kQS = 8;    // wavefronts of a residual row / column block (slices of the stored crosses)
 
 struct QAcaState {
-    double2 piv;
     double err;
-    int i, j, it, done;
-    int lim, pad0;
+    int i, it, done, lim;
 };
 
 // block first-max of (value, index); ties to the lower index; value -1 = none
@@ -110,59 +108,74 @@ __global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_row(const QAcaState *__re
         best = a == a ? a : -1.0, bi = c;  // NaN never wins
     }
     wave_argmax(best, bi);
-    if (lane == 0) pmax[2 * blockIdx.x] = best, pmax[2 * blockIdx.x + 1] = (double)bi;
-}
-
-// pivot column j = first max over the block partials (one wavefront; ties to the lower index, so
-// the result is the sequential first maximum); piv = v_j
-__global__ __launch_bounds__(ARH_WAVE) void k_qa_pivot(QAcaState *__restrict__ S, const double *__restrict__ pmax,
-                                                      int nb, int N, const double2 *__restrict__ V) {
-    if (S->done) return;
-    double bv = -1.0;
-    int bj = 0x7fffffff;
-    for (int b = threadIdx.x; b < nb; b += ARH_WAVE) {
-        const double v = pmax[2 * b];
-        const int j = (int)pmax[2 * b + 1];
-        if (v > bv || (v == bv && j < bj)) bv = v, bj = j;
-    }
-    wave_argmax(bv, bj);
-    if (threadIdx.x == 0) {
-        S->j = bj;
-        S->piv = V[(int64_t)S->it * N + bj];
+    // the partial carries v at the block's first maximum, so the pivot value is known without reading the
+    // row while it is being normalised
+    const int src = bi - (int)blockIdx.x * ARH_WAVE;
+    const int sl = src >= 0 && src < ARH_WAVE ? src : 0;
+    const double vr = __shfl(v.x, sl, ARH_WAVE), vi = __shfl(v.y, sl, ARH_WAVE);
+    if (lane == 0) {
+        double *o = pmax + 4 * blockIdx.x;
+        o[0] = best, o[1] = (double)bi, o[2] = vr, o[3] = vi;
     }
 }
 
-// V row `it` /= v_j (no zero-pivot guard, aca.cpp:104) + block partials of |v|^2
-__global__ __launch_bounds__(kQT) void k_qa_norm(const QAcaState *__restrict__ S, int N, double2 *__restrict__ V,
-                                                double *__restrict__ pvn) {
-    __shared__ double sv[kQT / ARH_WAVE];
-    const QAcaState st = *S;
-    if (st.done) return;
-    const int c = blockIdx.x * kQT + threadIdx.x;
-    double a = 0.0;
-    if (c < N) {
-        const double2 v = cdiv(V[(int64_t)st.it * N + c], st.piv);
-        V[(int64_t)st.it * N + c] = v;
-        a = cnorm2(v);
-    }
-    a = block_sum_q(a, sv);
-    if (threadIdx.x == 0) pvn[blockIdx.x] = a;
-}
-
-// residual column j into U column `it` + block partials of |u|^2 and of the masked first-max (64 rows a
-// block)
-__global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_col(const QAcaState *__restrict__ S,
-                                                          const double *__restrict__ L, const Tri *__restrict__ tri,
-                                                          double k, int LL, int N, double2 *__restrict__ U,
-                                                          const double2 *__restrict__ V,
-                                                          const unsigned char *__restrict__ used,
-                                                          double *__restrict__ pcol) {
+// One launch after the residual row, workgroups [0, nbn) normalising V row `it` (512 columns each, block
+// partials of |v|^2) and [nbn, nbn + nbc) forming the residual column j (64 rows each). Every workgroup
+// first finds the pivot column j = the first maximum over the row blocks' partials (ties to the lower
+// index, so it is the sequential first maximum) and takes v_j from the winning partial.
+__global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_normcol(const QAcaState *__restrict__ S,
+                                                              const double *__restrict__ L,
+                                                              const Tri *__restrict__ tri, double k, int LL, int N,
+                                                              double2 *__restrict__ U, double2 *__restrict__ V,
+                                                              const unsigned char *__restrict__ used,
+                                                              const double *__restrict__ pmax, int nbr, int nbn,
+                                                              double *__restrict__ pvn, double *__restrict__ pcol) {
     __shared__ double2 red[kQS - 1][ARH_WAVE];
+    __shared__ double sv[kQS];
+    __shared__ int sj;
+    __shared__ double2 spiv;
     const QAcaState st = *S;
     if (st.done) return;
-    const int j = st.j, it = st.it;
+    const int it = st.it;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / ARH_WAVE)), lane = threadIdx.x % ARH_WAVE;
-    const int l = blockIdx.x * ARH_WAVE + lane, ll = min(l, LL - 1);
+    if (w == 0) {
+        double bv = -1.0;
+        int bj = 0x7fffffff;
+        for (int b = lane; b < nbr; b += ARH_WAVE) {
+            const double v = pmax[4 * b];
+            const int j = (int)pmax[4 * b + 1];
+            if (v > bv || (v == bv && j < bj)) bv = v, bj = j;
+        }
+        wave_argmax(bv, bj);
+        if (lane == 0) {
+            const double *o = pmax + 4 * (bj / ARH_WAVE);  // the row block holding column bj
+            sj = bj, spiv = make_double2(o[2], o[3]);
+        }
+    }
+    __syncthreads();
+    const int j = sj;
+    if ((int)blockIdx.x < nbn) {
+        // V row `it` /= v_j (no zero-pivot guard, aca.cpp:104) + block partial of |v|^2
+        const int c = blockIdx.x * (kQS * ARH_WAVE) + threadIdx.x;
+        double a = 0.0;
+        if (c < N) {
+            const double2 v = cdiv(V[(int64_t)it * N + c], spiv);
+            V[(int64_t)it * N + c] = v;
+            a = cnorm2(v);
+        }
+        a = wave_sum(a);
+        if (lane == 0) sv[w] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = sv[0];
+            for (int q = 1; q < kQS; q++) t += sv[q];
+            pvn[blockIdx.x] = t;
+        }
+        return;
+    }
+    // residual column j into U column `it` + block partials of |u|^2 and of the masked first-max
+    const int bc = (int)blockIdx.x - nbn;
+    const int l = bc * ARH_WAVE + lane, ll = min(l, LL - 1);
     double2 u = make_double2(0.0, 0.0);
     if (w == 0) u = bem_entry(k, L[3 * ll], L[3 * ll + 1], L[3 * ll + 2], tri[j]);
     const double2 mine = cross_slice(it, w, U + ll, LL, V + j, N);
@@ -176,7 +189,7 @@ __global__ __launch_bounds__(kQS * ARH_WAVE) void k_qa_col(const QAcaState *__re
     }
     a = wave_sum(a);
     wave_argmax(best, bi);
-    if (lane == 0) pcol[3 * blockIdx.x] = a, pcol[3 * blockIdx.x + 1] = best, pcol[3 * blockIdx.x + 2] = (double)bi;
+    if (lane == 0) pcol[3 * bc] = a, pcol[3 * bc + 1] = best, pcol[3 * bc + 2] = (double)bi;
 }
 
 // Chunk partials of the non-conjugated products u^T u_p and v v_p^T with the stored crosses p < it
@@ -320,21 +333,20 @@ void qaca_build(arh_qaca *q, const double *L_host) {
     upload(q->L, hL, s);
     q->U.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)LL * lim));
     q->V.alloc(sizeof(double2) * std::max<int64_t>(1, (int64_t)lim * N));
-    q->nbr = (N + ARH_WAVE - 1) / ARH_WAVE;  // residual row blocks
-    q->nbn = (N + kQT - 1) / kQT;            // row normalisation blocks
-    q->nbc = (LL + ARH_WAVE - 1) / ARH_WAVE;  // residual column blocks
-    q->part.alloc(sizeof(double) * (2 * (size_t)q->nbr + q->nbn + 3 * (size_t)q->nbc + 16));
+    q->nbr = (N + ARH_WAVE - 1) / ARH_WAVE;               // residual row blocks
+    q->nbn = (N + kQS * ARH_WAVE - 1) / (kQS * ARH_WAVE);  // row normalisation blocks
+    q->nbc = (LL + ARH_WAVE - 1) / ARH_WAVE;               // residual column blocks
+    q->part.alloc(sizeof(double) * (4 * (size_t)q->nbr + q->nbn + 3 * (size_t)q->nbc + 16));
     const int ncu = (LL + kQC - 1) / kQC, ncv = (N + kQC - 1) / kQC;
     q->dots.alloc(sizeof(double2) * (size_t)std::max(1, lim) * (ncu + ncv));
     std::vector<unsigned char> used(LL, 0);
     used[0] = 1;
     upload(q->used, used, s);
     QAcaState st0{};
-    st0.piv = make_double2(1.0, 0.0);
-    st0.i = 0, st0.j = 0, st0.it = 0, st0.done = lim <= 0 ? 1 : 0, st0.lim = lim;
+    st0.i = 0, st0.it = 0, st0.done = lim <= 0 ? 1 : 0, st0.lim = lim;
     q->st.alloc(sizeof(QAcaState));
     ARH_HIP(hipMemcpyAsync(q->st.p, &st0, sizeof(QAcaState), hipMemcpyHostToDevice, s));
-    double *pmax = q->part.as<double>(), *pvn = pmax + 2 * q->nbr, *pcol = pvn + q->nbn;
+    double *pmax = q->part.as<double>(), *pvn = pmax + 4 * q->nbr, *pcol = pvn + q->nbn;
     QAcaState *S = q->st.as<QAcaState>();
     const Tri *tri = g->tri.as<Tri>();
     const double2 *Uc = q->U.as<double2>();
@@ -350,10 +362,9 @@ void qaca_build(arh_qaca *q, const double *L_host) {
             const int itb = launched + b;  // the cross index if none of the previous ones stopped
             hipLaunchKernelGGL(k_qa_row, dim3(q->nbr), dim3(kQS * ARH_WAVE), 0, s, S, q->L.as<double>(), tri, q->k,
                                LL, N, Uc, q->V.as<double2>(), pmax);
-            hipLaunchKernelGGL(k_qa_pivot, dim3(1), dim3(ARH_WAVE), 0, s, S, pmax, q->nbr, N, Vc);
-            hipLaunchKernelGGL(k_qa_norm, dim3(q->nbn), dim3(kQT), 0, s, S, N, q->V.as<double2>(), pvn);
-            hipLaunchKernelGGL(k_qa_col, dim3(q->nbc), dim3(kQS * ARH_WAVE), 0, s, S, q->L.as<double>(), tri, q->k,
-                               LL, N, q->U.as<double2>(), Vc, q->used.as<unsigned char>(), pcol);
+            hipLaunchKernelGGL(k_qa_normcol, dim3(q->nbn + q->nbc), dim3(kQS * ARH_WAVE), 0, s, S, q->L.as<double>(),
+                               tri, q->k, LL, N, q->U.as<double2>(), q->V.as<double2>(), q->used.as<unsigned char>(),
+                               pmax, q->nbr, q->nbn, pvn, pcol);
             if (itb > 0)
                 hipLaunchKernelGGL(k_qa_dots, dim3(itb, ncu + ncv), dim3(kQT), 0, s, S, LL, N, ncu, ncv, Uc, Vc, pu,
                                    pv);
