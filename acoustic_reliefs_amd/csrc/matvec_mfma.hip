@@ -675,17 +675,22 @@ __global__ __launch_bounds__(256) void k_fuse_panel(const int *__restrict__ leaf
                                                     const ColSrc *__restrict__ csrc, const double2 *__restrict__ dense,
                                                     const double2 *__restrict__ U, const double2 *__restrict__ V,
                                                     typename Mfma<T>::T2 *__restrict__ panel) {
+    // workgroup (leaf, y): its wavefronts take the leaf's panel columns y * 4 + w, + 4 gridDim.y, ...; the
+    // lanes of a wavefront take the rows of a column
     const int l = blockIdx.x;
     const int m = leaf_start[l + 1] - leaf_start[l];
     const LeafPanel L = lpanel[l];
-    const int64_t tot = (int64_t)L.ngroups * 4 * m;
-    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int c = (int)(e / m), i = (int)(e - (int64_t)c * m);
+    const int ncol = L.ngroups * 4;
+    const int w = threadIdx.x / ARH_WAVE, lane = threadIdx.x % ARH_WAVE;
+    for (int c = blockIdx.y * 4 + w; c < ncol; c += gridDim.y * 4) {
         const ColSrc cs = csrc[L.col0 + c];
         const double2 *src = cs.kind == 1 ? dense : cs.kind == 2 ? U : V;
-        double2 v = make_double2(0.0, 0.0);
-        if (cs.kind) v = src[cs.off + (int64_t)i * cs.stride];
-        panel[L.off + e] = Mfma<T>::make(v.x, v.y);
+        typename Mfma<T>::T2 *dst = panel + L.off + (int64_t)c * m;
+        for (int i = lane; i < m; i += ARH_WAVE) {
+            double2 v = make_double2(0.0, 0.0);
+            if (cs.kind) v = src[cs.off + (int64_t)i * cs.stride];
+            dst[i] = Mfma<T>::make(v.x, v.y);
+        }
     }
 }
 
@@ -867,7 +872,7 @@ static void build_copy(arh_hmat *h, int tr, int pi) {
                                M.d_maps.as<StackMap>(), h->V.as<double2>(), g->d_leaf_start.as<int>(),
                                g->d_leaf_of.as<int>(), M.vs[pi].as<T2>());
     }
-    hipLaunchKernelGGL(k_fuse_panel<T>, dim3(g->nleaf), dim3(256), 0, s, g->d_leaf_start.as<int>(),
+    hipLaunchKernelGGL(k_fuse_panel<T>, dim3(g->nleaf, 32), dim3(256), 0, s, g->d_leaf_start.as<int>(),
                        M.d_lpanel.as<LeafPanel>(), M.d_csrc.as<ColSrc>(), h->dense.as<double2>(), h->U.as<double2>(),
                        h->V.as<double2>(), M.panel[pi].as<T2>());
     ARH_HIP(hipGetLastError());
