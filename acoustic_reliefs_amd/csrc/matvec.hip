@@ -187,14 +187,15 @@ __global__ __launch_bounds__(kMvThreads) void k_panel_rowmajor(const int *__rest
     }
 }
 
-// ---------------------------------------------------------------- phase 2 (transposed): workgroup (column leaf, 64 columns)
+// ------------------------------------------------------ phase 2 (transposed): workgroup (column leaf, 64 columns)
 // W wavefronts take turns over the leaf's dense blocks and over the ranks of its low-rank blocks.
 constexpr int kColWaves = 16;
 template <class S, int W>
 __global__ __launch_bounds__(64 * W) void k_cols(const int *__restrict__ leaf_start, const int *__restrict__ dT_ptr,
                                                      const int *__restrict__ dT_blk, const int *__restrict__ dblk_info,
                                                      const int *__restrict__ pcol_ptr,
-                                                     const int64_t *__restrict__ panel_off, const S *__restrict__ panelT,
+                                                     const int64_t *__restrict__ panel_off,
+                                                     const S *__restrict__ panelT,
                                                      const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
                                                      const LRBlock *__restrict__ blocks, const S *__restrict__ V,
                                                      int vcm, const double2 *__restrict__ t,

@@ -194,9 +194,9 @@ __device__ __forceinline__ void grp_mma(Acc<T, NT, NRT> &acc, const Grp<T, NT, N
         for (int rt = 0; rt < NRT; rt++) acc[rt][nt] = Mfma<T>::mma(ai[rt], bi[nt], acc[rt][nt]);
 }
 
-// Load one column of a group: A column at acol (rows past amax clamped: their outputs are
-// dropped), xt row at brow (real values: the lane's output column is brow[16 nt]). No load is conditional, and nothing waits on a load before the group
-// is consumed (a padding mask is applied then).
+// Load one column of a group: A column at acol (rows past amax clamped: their outputs are dropped), xt row
+// at brow (real values: the lane's output column is brow[16 nt]). No load is conditional, and nothing waits
+// on a load before the group is consumed (a padding mask is applied then).
 template <class T, int NT, int NRT>
 __device__ __forceinline__ void grp_load(Grp<T, NT, NRT> &g, T okf, const typename Mfma<T>::T2 *acol, int amax,
                                          int i16, const T *brow) {
@@ -739,7 +739,8 @@ static void build_lists(arh_hmat *h, int tr) {
     // phase-1 units: big stacks first, each list by decreasing column count
     std::vector<int2> big, small;
     for (int k = 0; k < (int)M.stacks.size(); k++)
-        for (int rt = 0; rt * 32 < M.stacks[k].rtot; rt++) (M.stacks[k].n >= kBigCols ? big : small).push_back(int2{k, rt});
+        for (int rt = 0; rt * 32 < M.stacks[k].rtot; rt++)
+            (M.stacks[k].n >= kBigCols ? big : small).push_back(int2{k, rt});
     auto by_cols = [&](const int2 &p, const int2 &q) {
         const int np = M.stacks[p.x].n, nq = M.stacks[q.x].n;
         return np != nq ? np > nq : (p.x != q.x ? p.x < q.x : p.y < q.y);
