@@ -10,7 +10,10 @@ namespace arh {
 constexpr int kAcaThreads = 256;
 constexpr int kAcaWaves = kAcaThreads / ARH_WAVE;
 constexpr int kMaxRankLds = 128;  // max ACA rank kept in LDS coefficient arrays
-constexpr int kAcaBigDim = 1024;  // blocks with m + n at least this get a workgroup, smaller ones a wavefront
+#ifndef ARH_ACA_BIGDIM
+#define ARH_ACA_BIGDIM 1024
+#endif
+constexpr int kAcaBigDim = ARH_ACA_BIGDIM;  // blocks with m + n at least this get a workgroup, smaller ones a wavefront
 
 // ------------------------------------------------------------------ dense panels
 // One thread per entry of a (row leaf, column chunk) task; entries of a column are contiguous,
