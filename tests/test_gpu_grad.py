@@ -100,3 +100,28 @@ def test_gradient_fd_gpu(setup, capi):
         xm[t] -= h
         fd = (d.value(xp) - d.value(xm)) / (2 * h)
         assert abs(fd - grad[t]) < 2e-5 * max(abs(fd), 1e-3), (t, fd, grad[t])
+
+
+def test_diffbem_cluster128_vs_oracle(ctx, capi, oracle, small_relief):
+    """DiffBEM at the reference optimiser's cluster_size 128 (leaves of 65-128 elements take the MFMA engine's
+    8-row-tile class for the forward and the transposed products) against the oracle's band pipeline on the same
+    partition."""
+    from acoustic_reliefs_amd import acoustics3d_impl as m
+    P, E, top = small_relief
+    d = m.DiffBEM(128, 1.5, [1000.0], 1, 1e-8, 1e-8, 1e-10, np.array([0.0, 100.0, 0.0]), 50, 15, False,
+                  speed_of_sound=343.0)
+    d.silent = True
+    d.precompute(P, E, top)
+    x = 0.01 * np.sin(np.arange(len(top)))
+    print("DiffBEM cluster_size 128", flush=True)
+    val, grad = d.gradient(x)
+    Er, dirb, appb, _ = oracle.setup(P, E, 128, 1.5)
+    assert max(int(b[1] - b[0]) for b in dirb) > 64
+    Pd = P.copy()
+    Pd[top, 1] += x
+    oc = oracle.Ctx(Pd, Er, dirb, appb)
+    vo, go = oracle.band_value_grad(oc, [2 * np.pi * 1000.0 / 343.0], d.get_listeners(), top, aca_tol=1e-8,
+                                    solver_tol=1e-10, want_grad=True, q_aca_tol=1e-8)
+    print("value %.10f oracle %.10f; grad rel diff %.2e" % (val, vo, rel(grad, go)))
+    assert abs(val - vo) < 1e-5
+    assert rel(grad, go) < 1e-3
