@@ -15,9 +15,14 @@
 //   entry / block / row / col   : k=0 Gauss-law sums, degree-4 exactness of the rule,
 //                                 adaptive brute-force quadrature, M_ii == 0
 //   tree / partition            : ball check, exact cover of N^2, permutation
-//   ACA                         : SVD error bound, exact recovery of low-rank blocks
+//   ACA                         : SVD error bound, exact recovery of synthetic rank-r blocks
+//                                 (rank exactly r), a hand-traced 3x3 complex cross sequence
+//                                 (non-conjugated estimate, discarded stopping cross)
 //   H-matvec                    : dense brute force (M - I/2) x
-//   BiCGSTAB                    : dense LU solution, residual <= tol, hand-checked 2x2 trace
+//   BiCGSTAB                    : dense LU solution, residual <= tol, finite termination on
+//                                 matrices with 1, 2 and 3 distinct eigenvalues
+//   right-hand sides            : point source -> plane wave in the far field, by hand near by
+//   listener points, bands      : entries by hand, closed form of the log spacing
 //   diffusion coefficient / u   : closed forms (uniform -> 1, single receiver -> 0), FD of u
 //   derivatives / gradient      : central FD of compute_M, FD of the whole forward
 //   full solve                  : analytic rigid-sphere series
@@ -791,6 +796,47 @@ int orc_bicgstab(void *h, int transpose, const double *b, double *x, double tol,
         if (maxit < 0) maxit = 2 * N;  // Eigen default: 2 * cols
         std::vector<double> hs;
         auto op = [&](const cd *xx, cd *yy) { hmat_matvec(c->H, transpose != 0, xx, yy); };
+        *iters = bicgstab(N, op, (const cd *)b, (cd *)x, tol, maxit, hs, *err);
+        if (hist) std::copy(hs.begin(), hs.end(), hist);
+    });
+}
+
+// The block ACA (aca.cpp:38-82; guard = 1) or the listener ACA's unguarded pivot division
+// (aca.cpp:104; guard = 0) on an explicit m x n matrix A (column-major, interleaved complex), so
+// the cross sequence can be checked on matrices whose crosses are known by hand. U (m x lim
+// column-major) and V (lim x n row-major) with lim = min(max_iter, m - 1, n - 1) are written
+// when non-null; *rank receives the number of crosses kept.
+int orc_aca_dense(int m, int n, const double *A, double eps, int max_iter, int guard, double *U, double *V,
+                  int *rank) {
+    return guarded([&] {
+        const cd *Am = (const cd *)A;
+        auto row = [&](int i, cd *out) {
+            for (int c = 0; c < n; c++) out[c] = Am[(size_t)c * m + i];
+        };
+        auto col = [&](int j, cd *out) {
+            for (int r = 0; r < m; r++) out[r] = Am[(size_t)j * m + r];
+        };
+        std::vector<cd> Uv, Vv;
+        *rank = aca(m, n, eps, max_iter, row, col, guard != 0, Uv, Vv);
+        if (U) std::memcpy(U, Uv.data(), Uv.size() * sizeof(cd));
+        if (V) std::memcpy(V, Vv.data(), Vv.size() * sizeof(cd));
+    });
+}
+
+// BiCGSTAB (the restatement used on the H-matrix) on an explicit N x N matrix A (column-major,
+// interleaved complex), so finite termination and hand-traced recurrences can pin it.
+// maxit < 0: 2 N. hist capacity maxit + 2.
+int orc_bicgstab_dense(int N, const double *A, const double *b, double *x, double tol, int maxit, double *hist,
+                       int *iters, double *err) {
+    return guarded([&] {
+        const cd *Am = (const cd *)A;
+        if (maxit < 0) maxit = 2 * N;
+        auto op = [&](const cd *xx, cd *yy) {
+            for (int i = 0; i < N; i++) yy[i] = 0;
+            for (int j = 0; j < N; j++)
+                for (int i = 0; i < N; i++) yy[i] += Am[(size_t)j * N + i] * xx[j];
+        };
+        std::vector<double> hs;
         *iters = bicgstab(N, op, (const cd *)b, (cd *)x, tol, maxit, hs, *err);
         if (hist) std::copy(hs.begin(), hs.end(), hist);
     });

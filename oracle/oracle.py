@@ -33,7 +33,7 @@ def lib():
         L.orc_ctx_destroy.argtypes = [c_void_p]
         for name in ["orc_set_vertices", "orc_assemble", "orc_ranks", "orc_factors", "orc_dense_block", "orc_matvec",
                      "orc_bicgstab", "orc_dense_matrix", "orc_listener_matrix", "orc_aca_listener", "orc_rhs",
-                     "orc_shape_grad"]:
+                     "orc_shape_grad", "orc_aca_dense", "orc_bicgstab_dense"]:
             getattr(L, name).restype = c_int
         L.orc_set_vertices.argtypes = [c_void_p, P_D]
         L.orc_assemble.argtypes = [c_void_p, c_double, c_double, c_int]
@@ -53,6 +53,8 @@ def lib():
         L.orc_dentry.argtypes = [c_double, P_D, P_D, c_int, P_D]
         L.orc_partition.argtypes = [P_D, c_int, c_int, c_double, P_I, P_I, P_I, P_I, P_I, c_int, P_I, P_I]
         L.orc_tree_check.argtypes = [P_D, c_int, c_int]
+        L.orc_aca_dense.argtypes = [c_int, c_int, P_D, c_double, c_int, c_int, P_D, P_D, P_I]
+        L.orc_bicgstab_dense.argtypes = [c_int, P_D, P_D, P_D, c_double, c_int, P_D, P_I, P_D]
         L.orc_dense_solve.argtypes = [c_int, P_D, P_D, P_D]
         L.orc_diffusion.restype = c_double
         L.orc_diffusion.argtypes = [c_int, P_D, P_D]
@@ -156,6 +158,38 @@ def dense_solve(A, b):
     x = np.zeros(N, np.complex128)
     _check(lib().orc_dense_solve(N, _d(flat.view(np.float64)), _d(b.view(np.float64)), _d(x.view(np.float64))))
     return x
+
+
+def aca_dense(A, eps, max_iter=100, guard=True):
+    """Partially pivoted ACA (aca.cpp:38-82; guard=False: the listener variant's unguarded pivot
+    division, aca.cpp:104) of an explicit m x n matrix. Returns U (m x r), V (r x n)."""
+    A = np.asarray(A, dtype=np.complex128)
+    m, n = A.shape
+    lim = max(0, min(max_iter, m - 1, n - 1))
+    flat = np.ascontiguousarray(A.reshape(-1, order="F"))
+    U = np.zeros(max(1, m * lim), np.complex128)
+    V = np.zeros(max(1, lim * n), np.complex128)
+    r = c_int()
+    _check(lib().orc_aca_dense(m, n, _d(flat.view(np.float64)), float(eps), int(max_iter), int(guard),
+                               _d(U.view(np.float64)), _d(V.view(np.float64)), ctypes.byref(r)))
+    r = r.value
+    return U[:m * r].reshape(r, m).T, V[:r * n].reshape(r, n)
+
+
+def bicgstab_dense(A, b, tol, maxit=-1):
+    """The oracle's BiCGSTAB (Eigen 3.4 semantics) on an explicit matrix: x, iterations, error, history."""
+    A = np.asarray(A, dtype=np.complex128)
+    N = A.shape[0]
+    flat = np.ascontiguousarray(A.reshape(-1, order="F"))
+    b = _c128(b)
+    x = np.zeros(N, np.complex128)
+    mi = 2 * N if maxit < 0 else maxit
+    hist = np.zeros(mi + 2)
+    it = c_int()
+    err = c_double()
+    _check(lib().orc_bicgstab_dense(N, _d(flat.view(np.float64)), _d(b.view(np.float64)), _d(x.view(np.float64)),
+                                    float(tol), int(maxit), _d(hist), ctypes.byref(it), ctypes.byref(err)))
+    return x, it.value, err.value, hist[:it.value + 1].copy()
 
 
 def diffusion(yt):

@@ -71,3 +71,39 @@ def test_new_entry_points_reject_bad_arguments(capi):
     b = capi.c_size_t()
     assert lib.arh_device_cached(0, capi.ctypes.byref(b)) == 0 and b.value == 0
     assert lib.arh_device_cached(0, None) != 0
+
+
+def _line(xs):
+    return np.column_stack([np.asarray(xs, dtype=np.float64), np.zeros(len(xs)), np.zeros(len(xs))])
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_partition_by_hand(impl, capi):
+    """Cluster tree + dual traversal (clustertree.cpp:60-137) traced by hand on centroids on the x axis,
+    input order x = (3, 10, 0, 2, 1), leaf 2, factor 1.5:
+
+    root [0,5): mean 3.2, radius 6.8; x < 3.2 left in input order: ids (0, 2, 3, 4), right (1).
+    L [0,4): mean 1.5, radius 1.5 -> LL = ids (2, 4), LR = ids (0, 3); R [4,5): id 1, radius 0.
+    perm = (2, 4, 0, 3, 1). Traversal from (root, root): equal sizes split the target; a leaf target
+    splits the source; admissible when dist > 1.5 (r_s + r_t):
+      (LL,LL) direct; (LR,LL) dist 2 > 1.5 approx; (LL,LR) approx; (LR,LR) direct; (R,L) dist 8.5 > 2.25
+      approx; (L,R) approx; (R,R) direct.
+    The product's host partition (arh_partition) and the oracle must both give exactly this."""
+    if impl == "oracle":
+        from oracle import oracle as O
+        perm, d, a, _ = O.partition(_line([3, 10, 0, 2, 1]), 2, 1.5)
+    else:
+        perm, d, a = capi.partition(_line([3, 10, 0, 2, 1]), 2, 1.5)
+    assert perm.tolist() == [2, 4, 0, 3, 1]
+    assert d.tolist() == [[0, 2, 0, 2], [2, 4, 2, 4], [4, 5, 4, 5]]
+    assert a.tolist() == [[2, 4, 0, 2], [0, 2, 2, 4], [4, 5, 0, 4], [0, 4, 4, 5]]
+    # a centroid exactly at the mean goes right ('<' goes left): x = (0, 1, 2), leaf 2 -> {0} | {1, 2}
+    if impl == "oracle":
+        perm, d, a, _ = O.partition(_line([2, 1, 0]), 2, 1.5)
+    else:
+        perm, d, a = capi.partition(_line([2, 1, 0]), 2, 1.5)
+    assert perm.tolist() == [2, 0, 1]
+    # (root,root) splits the target; (root,{0}) splits the source: ({0},{0}) direct, ({1,2},{0}) centres
+    # 1.5 apart > 1.5 (0.5 + 0) = 0.75: approx; (root,{1,2}) splits the source: ({0},{1,2}) approx,
+    # ({1,2},{1,2}) direct
+    assert d.tolist() == [[0, 1, 0, 1], [1, 3, 1, 3]] and a.tolist() == [[1, 3, 0, 1], [0, 1, 1, 3]]

@@ -128,6 +128,67 @@ def test_aca_error_bound_and_svd(oracle, sphere_mesh):
                 assert err >= 0.99 * sv[ranks[blk]] / np.linalg.norm(sv) - 1e-15
 
 
+def test_aca_hand_traced_cross_sequence(oracle):
+    """The block ACA (aca.cpp:38-82) traced by hand on A = [[-1, -i, -1], [1, 1+i, 1+i], [-i, 1+i, -i]]:
+
+    cross 1: start row 0 = (-1, -i, -1), |.|^2 = (1, 1, 1): first maximum, j = 0; v1 = row / (-1) =
+             (1, i, 1); u1 = column 0 = (-1, 1, -i); |u1|^2 |v1|^2 = 3 * 3 = 9 = err, criterion 1.
+    cross 2: next row = first maximum of |u1|^2 over unused rows 1, 2 (1, 1): row 1. Residual row
+             (1, 1+i, 1+i) - 1 * v1 = (0, 1, i): j = 1 (first maximum), v2 = (0, 1, i);
+             u2 = (-i, 1+i, 1+i) - i u1 = (0, 1, i); |u2|^2 |v2|^2 = 2 * 2 = 4.
+             The estimate's products are NOT conjugated (aca.cpp:66-67): u2^T u1 = 1 + (-i)(i) = 2 and
+             v2 . v1 = i + i = 2i, so err = 9 + 4 + 2 * 2 * 2 = 21 and the criterion is sqrt(4 / 21) = 0.436.
+             Conjugating either product gives 0 (u1^H u2 = 1 - 1, v2 . conj(v1) = -i + i): err = 13 and
+             the criterion would be sqrt(4 / 13) = 0.555.
+    lim = min(max, m - 1, n - 1) = 2 crosses at most (aca.cpp:44).
+    """
+    A = np.array([[-1, -1j, -1], [1, 1 + 1j, 1 + 1j], [-1j, 1 + 1j, -1j]], dtype=np.complex128)
+    # eps = 0.5 lies between the two criteria: the reference stops at cross 2 and discards it
+    # (aca.cpp:71, 78-79): rank 1; a conjugated estimate would keep iterating (rank 2)
+    U, V = oracle.aca_dense(A, 0.5)
+    assert U.shape == (3, 1) and V.shape == (1, 3)
+    assert np.array_equal(U[:, 0], [-1, 1, -1j]) and np.array_equal(V[0], [1, 1j, 1])
+    # eps = 0.4 < 0.436: both crosses kept, with the values traced above
+    U, V = oracle.aca_dense(A, 0.4)
+    assert U.shape == (3, 2)
+    assert np.array_equal(U[:, 1], [0, 1, 1j]) and np.array_equal(V[1], [0, 1, 1j])
+    # the cross approximation interpolates its pivot rows (0, 1) and columns (0, 1)
+    R = A - U @ V
+    assert np.abs(R[:2]).max() == 0 and np.abs(R[:, :2]).max() == 0
+    # max_iter caps the crosses
+    assert oracle.aca_dense(A, 0.4, max_iter=1)[0].shape[1] == 1
+    # the listener variant (no zero-pivot guard, aca.cpp:104) takes the same crosses here
+    assert np.array_equal(oracle.aca_dense(A, 0.4, guard=False)[1], V)
+
+
+def test_aca_exact_recovery_of_low_rank_blocks(oracle):
+    """A rank-r block is recovered with exactly r crosses: the (r+1)-th residual vanishes, meets any
+    tolerance and is discarded (aca.cpp:71, 78-79), so keeping the stopping cross would give r + 1.
+    1 x n and m x 1 blocks get rank 0 (iter_lim = min(max, m - 1, n - 1), aca.cpp:44)."""
+    rng = np.random.default_rng(7)
+    for m, n, r in ((40, 30, 5), (64, 64, 1), (33, 50, 12)):
+        X = rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))
+        Y = rng.standard_normal((r, n)) + 1j * rng.standard_normal((r, n))
+        A = X @ Y
+        for guard in (True, False):
+            U, V = oracle.aca_dense(A, 1e-10, guard=guard)
+            assert U.shape[1] == r, (m, n, r, U.shape)
+            assert np.linalg.norm(A - U @ V) <= 1e-12 * np.linalg.norm(A)
+            # start row 0 reproduced; each V row has a unit entry at its pivot and nothing larger
+            assert np.linalg.norm((A - U @ V)[0]) <= 1e-13 * np.linalg.norm(A[0])
+            for p in range(r):
+                assert abs(np.abs(V[p]).max() - 1.0) < 1e-14
+        U, _ = oracle.aca_dense(A, 1e-10, max_iter=r - 1 if r > 1 else 1)
+        assert U.shape[1] == (r - 1 if r > 1 else 1)
+    row = rng.standard_normal((1, 20)) + 0j
+    assert oracle.aca_dense(row, 1e-10)[0].shape[1] == 0
+    assert oracle.aca_dense(row.T.copy(), 1e-10)[0].shape[1] == 0
+    # an all-zero block: the guarded pivot (aca.cpp:56-60) keeps v as is; the criterion 0/0 is not
+    # <= eps, so the crosses run to iter_lim and contribute exact zeros
+    U, V = oracle.aca_dense(np.zeros((5, 4), np.complex128), 1e-3)
+    assert U.shape[1] == 3 and not np.any(U @ V)
+
+
 def test_listener_aca_pinned(oracle, small_relief):
     """Listener ACA (aca.cpp:84-134) vs the exact listener matrix: error bound, optimal-rank floor,
     exact interpolation of row 0 (the start row) and of every pivot column, the rank limit
@@ -193,6 +254,97 @@ def test_bicgstab_matches_dense_and_tolerance(oracle, sphere_mesh):
     # zero right-hand side: x = 0 without iterating (Eigen's early exit)
     x, it, err, _ = oc.bicgstab(np.zeros_like(b), 1e-6)
     assert it == 0 and not np.any(x)
+
+
+def _similar(rng, eigs):
+    """A diagonalisable matrix P diag(eigs) P^-1 with a well-conditioned complex P."""
+    N = len(eigs)
+    P = np.eye(N) + 0.2 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / np.sqrt(N)
+    return P @ np.diag(eigs) @ np.linalg.inv(P)
+
+
+def test_bicgstab_finite_termination(oracle):
+    """BiCGSTAB's residual after iteration i is Q_i(A) P_i(A) r0 with P_i the BiCG polynomial, so in
+    exact arithmetic it ends after as many iterations as A has distinct eigenvalues (with r0 generic).
+    A wrong beta = (rho/rho_old)(alpha/omega), alpha = rho/(r0^H v), omega = (t^H s)/|t|^2 or update
+    order breaks this (the Eigen 3.4 recurrence the oracle restates)."""
+    rng = np.random.default_rng(11)
+    N = 12
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    # A = lambda I: s = r - alpha v = 0 in the first iteration, t = 0, omega = 0, x = b / lambda
+    lam = 0.7 - 1.3j
+    x, it, err, hist = oracle.bicgstab_dense(lam * np.eye(N), b, 1e-13)
+    assert it == 1 and err <= 1e-15 and hist[0] == 1.0
+    assert np.linalg.norm(x - b / lam) <= 1e-15 * np.linalg.norm(b / lam)
+    for eigs, want in (([2.0 + 0.5j] * 5 + [-1.0 + 1j] * 7, 2), ([1.5] * 4 + [0.5 - 2j] * 4 + [-3 + 0.2j] * 4, 3)):
+        A = _similar(rng, np.array(eigs, dtype=np.complex128))
+        x, it, err, hist = oracle.bicgstab_dense(A, b, 1e-12)
+        assert it == want, (want, it, hist)
+        assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b)
+        # not converged one iteration earlier
+        assert hist[want - 1] > 1e-3
+    # a generic matrix needs the full dimension or close to it, and stops on the recursive residual
+    A = _similar(rng, rng.standard_normal(N) + 1j * rng.standard_normal(N) + 4)
+    x, it, err, hist = oracle.bicgstab_dense(A, b, 1e-10)
+    assert it <= N and err <= 1e-10 and abs(err - hist[-1]) == 0
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+    # maxit caps the iterations (Eigen's setMaxIterations)
+    assert oracle.bicgstab_dense(A, b, 1e-14, maxit=3)[1] == 3
+
+
+def test_point_source_far_field_is_plane_wave(oracle, sphere_mesh):
+    """b_i = e^{-ik|c_i - s|} / (4 pi |c_i - s|) (helpers.cpp:46-50) for a source s = R d far away:
+    4 pi R e^{ikR} b_i -> e^{ik d.c_i}, the plane wave e^{-ik d'.c_i} travelling along d' = -d
+    (sphere.h:22-26 convention), with error O(k |c|^2 / R)."""
+    P, E = sphere_mesh
+    Er, dirb, appb, _ = oracle.setup(P, E, 32, 1.5)
+    oc = oracle.Ctx(P, Er, dirb, appb)
+    d = np.array([0.3, 0.8, -0.52])
+    d /= np.linalg.norm(d)
+    for R in (1e5, 1e7):
+        bp = oc.rhs_point(K500, R * d) * 4 * np.pi * R * np.exp(1j * K500 * R)
+        bw = oc.rhs_plane(K500, -d)
+        err = np.abs(bp - bw).max()
+        assert err < 5 * K500 * 0.5 ** 2 / R + 1e-7, (R, err)
+    # near field, one point by hand: c = centroid 0, |c - s| = r
+    C, _ = oracle.intermediates(P, Er)
+    s = np.array([0.0, 100.0, 0.0])
+    r = np.linalg.norm(C[0] - s)
+    want = np.exp(-1j * K500 * r) / (4 * np.pi * r)
+    assert abs(oc.rhs_point(K500, s)[0] - want) < 1e-12 * abs(want)  # k r = 916 rad: phase rounding ~1e-13
+
+
+def test_listener_points_by_hand(oracle):
+    """compute_listener_pts (helpers.cpp:34-65) with r = 50, ds = 5: the pole first, then the
+    elevation rings 5..90 with azimuths 0..355 (72 each): 1 + 18 * 72 = 1297 points, each
+    (r sin t cos p, r cos t, -r sin t sin p) with t the elevation from +y."""
+    L = oracle.listener_points(50.0, 5.0)
+    assert L.shape == (1297, 3)
+    s5, c5 = 0.08715574274765817, 0.9961946980917455  # sin 5 deg, cos 5 deg
+    np.testing.assert_allclose(L[0], [0.0, 50.0, 0.0], atol=1e-15)
+    np.testing.assert_allclose(L[1], [50 * s5, 50 * c5, 0.0], rtol=0, atol=1e-14)     # t = 5, p = 0
+    np.testing.assert_allclose(L[19], [0.0, 50 * c5, -50 * s5], rtol=0, atol=1e-14)   # t = 5, p = 90
+    np.testing.assert_allclose(L[73], [50 * np.sin(np.pi / 18), 50 * np.cos(np.pi / 18), 0.0], atol=1e-14)  # t = 10
+    np.testing.assert_allclose(L[-1], [50 * np.cos(np.radians(355)), 0.0, -50 * np.sin(np.radians(355))],
+                               atol=1e-13)  # t = 90, p = 355
+    np.testing.assert_allclose(np.linalg.norm(L, axis=1), 50.0, rtol=1e-15)
+    assert np.all(L[:, 1] >= -1e-12)  # upper hemisphere (y up)
+    # ds = 15: 1 + 6 * 24
+    assert len(oracle.listener_points(10.0, 15.0)) == 145
+
+
+def test_band_frequencies_closed_form(oracle):
+    """get_frequencies (helpers.cpp:67-83): n log-spaced frequencies over F 2^(+-1/6), one third of
+    an octave: ends F 2^(-1/6) and F 2^(1/6), geometric mean F, constant ratio; n = 1 gives F."""
+    assert oracle.frequencies(1000.0, 1) == [1000.0]
+    for F in (250.0, 1000.0, 3150.0):
+        for n in (2, 3, 5):
+            f = np.array(oracle.frequencies(F, n))
+            assert len(f) == n
+            assert abs(f[0] - F * 2 ** (-1 / 6)) < 1e-12 * F and abs(f[-1] - F * 2 ** (1 / 6)) < 1e-12 * F
+            assert abs(np.exp(np.mean(np.log(f))) - F) < 1e-12 * F
+            np.testing.assert_allclose(f[1:] / f[:-1], 2 ** (1 / (3 * (n - 1))), rtol=1e-13)
+        assert abs(oracle.frequencies(F, 3)[1] - F) < 1e-12 * F
 
 
 def test_diffusion_closed_forms(oracle):
