@@ -17,6 +17,10 @@ Backward (adjoint.cpp:135-239, batched over the incidences):
 recompute=True frees each band's H-matrix after the forward and re-assembles it in the backward (the
 reference's recompute_matrices, adjoint.cpp:104-109, 173-193), trading assembly time for HBM.
 
+Every forward call returns its own state (heights, per-band solutions, adjoint weights and kept
+operators), and the backward of an output uses the state of the forward that produced it: further
+forwards in between (a line search, finite-difference probes, gradcheck) do not change it.
+
 Heights are the y offsets of the differentiable vertices (the relief's top grid), as in
 DiffBEM.gradient (acoustics3d.cpp:210-213). Everything runs through the C-ABI (capi.py).
 """
@@ -44,7 +48,7 @@ class ReliefBEM:
         self.dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 3)
         self.L = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
         self.c0, self.aca_tol, self.tol, self.storage, self.recompute = c0, aca_tol, tol, storage, recompute
-        self._saved = None
+        self.last_state = None
         self.last_stats = []
         self.timing = []  # per band: assemble / solve / receivers (forward), adjoint / grad (backward), ms
         self.log = None   # optional callable(str): a line per band as it completes
@@ -58,7 +62,14 @@ class ReliefBEM:
         self.g.set_vertices(P)
 
     def forward(self, h):
-        """Per-band mean diffusion coefficient over the incidences (and the per-incidence values)."""
+        """Per-band mean diffusion coefficient over the incidences (and the per-incidence values).
+        The call's state is kept as last_state for backward(grad_d)."""
+        d_b, d_all, _ = self.forward_state(h)
+        return d_b, d_all
+
+    def forward_state(self, h):
+        """forward(h) that also returns the call's state, for backward(grad_d, state)."""
+        h = np.array(h, dtype=np.float64, copy=True)
         self.set_heights(h)
         R, LL = len(self.dirs), len(self.L)
         saved, d_b, d_all = [], [], []
@@ -88,15 +99,21 @@ class ReliefBEM:
                                 "receivers_ms": t_rec, "iters": st["iters"], "hmat_bytes": info["bytes_total"]})
             if self.log:
                 self.log("forward band %g Hz: %s" % (f, self.timing[-1]))
-        self._saved = saved
-        return np.array(d_b), np.array(d_all)
+        state = {"h": h, "bands": saved}
+        self.last_state = state
+        return np.array(d_b), np.array(d_all), state
 
-    def backward(self, grad_d):
-        """dL/dh from dL/dd_b (one value per band)."""
-        assert self._saved is not None, "forward must run first"
+    def backward(self, grad_d, state=None):
+        """dL/dh from dL/dd_b (one value per band) at the heights of state's forward (default: the
+        latest forward)."""
+        state = state if state is not None else self.last_state
+        if state is None:
+            raise RuntimeError("forward must run first")
+        # the shape gradient (and a recomputed operator) need the geometry at this forward's heights
+        self.set_heights(state["h"])
         R, LL, HH = len(self.dirs), len(self.L), len(self.diff)
         g = np.zeros(HH)
-        for b, s in enumerate(self._saved):
+        for b, s in enumerate(state["bands"]):
             w = float(grad_d[b]) / R
             if w == 0.0:
                 continue
@@ -129,14 +146,15 @@ class DiffusionCoefficients(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, heights, engine):
-        d_b, _ = engine.forward(heights.detach().cpu().double().numpy())
+        d_b, _, state = engine.forward_state(heights.detach().cpu().double().numpy())
         ctx.engine = engine
+        ctx.state = state
         ctx.hdev = heights.device
         return torch.from_numpy(d_b).to(heights.device)
 
     @staticmethod
     def backward(ctx, grad_d):
-        g = ctx.engine.backward(grad_d.detach().cpu().double().numpy())
+        g = ctx.engine.backward(grad_d.detach().cpu().double().numpy(), ctx.state)
         return torch.from_numpy(g).to(ctx.hdev), None
 
 

@@ -8,6 +8,7 @@ without a GPU, using the library needs one.
 import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -19,7 +20,8 @@ ARH_FP64 = 0
 ARH_MIXED = 1
 ARH_RHS_POINT = 0
 ARH_RHS_PLANE = 1
-ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENAN", -4: "ECONV", -5: "ENOMEM", -6: "ESTATE"}
+ARH_ECANCEL = -7
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENAN", -4: "ECONV", -5: "ENOMEM", -6: "ESTATE", -7: "ECANCEL"}
 
 c_int, c_double, c_void_p, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
 P_D = ctypes.POINTER(ctypes.c_double)
@@ -130,20 +132,27 @@ SIGNATURES = [
     ("arh_server_poll", c_int, [c_void_p, ctypes.c_int64, ctypes.POINTER(RequestInfo)]),
     ("arh_server_profile", c_int, [c_void_p, c_int]),
     ("arh_server_get_profile", c_int, [c_void_p, ctypes.POINTER(Profile)]),
+    ("arh_server_cancel", c_int, [c_void_p, P_I]),
     ("arh_server_destroy", None, [c_void_p]),
 ]
 
 _lib = None
 _EXITING = False
+_LIVE_SERVERS = weakref.WeakSet()
 
 
-def _mark_exit():
-    # handles still alive at interpreter exit are left to the process teardown
+def _at_exit():
+    """Servers still alive at interpreter exit are destroyed first: their queued requests are
+    cancelled, running ones finish and the worker threads are joined while the HIP runtime is still
+    up (worker threads issuing HIP work during the runtime's own teardown crash the process). Other
+    handles hold no threads and are left to the process teardown."""
     global _EXITING
+    for srv in list(_LIVE_SERVERS):
+        srv.close()
     _EXITING = True
 
 
-atexit.register(_mark_exit)
+atexit.register(_at_exit)
 
 
 def lib():
@@ -538,7 +547,10 @@ class HMatrix:
 
 
 class Server:
-    """Continuously batched band solves (arh_server_*): submit returns at once, wait blocks."""
+    """Continuously batched band solves (arh_server_*): submit returns at once, wait blocks.
+
+    close() (also run for every live server at interpreter exit) cancels what is still queued, lets
+    running requests finish and joins the worker threads."""
 
     def __init__(self, workers=4, hbm_budget=0, device=0, maxit=0):
         h = c_void_p()
@@ -546,6 +558,7 @@ class Server:
         _check(lib().arh_server_create(ctypes.byref(opts), ctypes.byref(h)))
         self.h = h
         self._nrhs = {}
+        _LIVE_SERVERS.add(self)
 
     def add_mesh(self, P, E, leaf=32, factor=1.5):
         P = np.ascontiguousarray(P, dtype=np.float64)
@@ -575,6 +588,12 @@ class Server:
         _check(lib().arh_server_poll(self.h, ctypes.c_int64(rid), ctypes.byref(info)))
         return info.as_dict()
 
+    def cancel(self):
+        """Drop the queued requests (they end with status ARH_ECANCEL); returns how many."""
+        n = c_int()
+        _check(lib().arh_server_cancel(self.h, ctypes.byref(n)))
+        return n.value
+
     def profile(self, enable=True):
         _check(lib().arh_server_profile(self.h, int(enable)))
 
@@ -584,9 +603,11 @@ class Server:
         return p.as_dict()
 
     def close(self):
-        if getattr(self, "h", None) and not _EXITING:
-            lib().arh_server_destroy(self.h)
-        self.h = None
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            _LIVE_SERVERS.discard(self)
+            lib().arh_server_destroy(h)
 
     def __del__(self):
-        self.close()
+        if not _EXITING:
+            self.close()

@@ -931,12 +931,16 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
         ARH_HIP(hipEventRecord(e.b, s));
         pf.pending.push_back(e);
     };
-    const double eb = 2.0 * sizeof(T);  // bytes of one complex operator entry
+    // Algorithmic bytes: every operator entry once (eb = one complex entry of T) and every packed rhs
+    // / intermediates value once per rhs (also eb: xt holds T2 values), plus the FP64 x read for the
+    // -x/2 term and the FP64 y write (16 B per rhs per element) in phase 2. Partial tiles of split
+    // leaves and re-reads are not algorithmic.
+    const double eb = 2.0 * sizeof(T);
     const int nu = (int)M.p1units.size(), nb = M.p1big;
     // phase-1 entries: V (forward) or U (transposed); both total sum_b r_b (m_b or n_b)
     const int64_t e1 = M.vs_total;
     if (nu)
-        timed(pi ? PROF_LR64 : PROF_LR32, eb * e1 + 16.0 * R * (N + h->t_total), 8.0 * R * e1, [&] {
+        timed(pi ? PROF_LR64 : PROF_LR32, eb * e1 + eb * R * (N + h->t_total), 8.0 * R * e1, [&] {
             hipLaunchKernelGGL((k_lr_mfma<T, NT>), dim3(nb + (nu - nb + kLrWaves - 1) / kLrWaves),
                                dim3(64 * kLrWaves), 0, s, M.d_stacks.as<VStack>(), M.d_p1units.as<int2>(), nb, nu,
                                M.vs[pi].as<T2>(), xt, gate);
@@ -946,7 +950,7 @@ static void launch_nt(arh_hmat *h, MfmaLayout &M, int R, const double2 *x, doubl
     const int ntask = (int)M.rtasks.size(), nsp = (int)M.rsplit.size();
     const double *xd = (const double *)x;
     double *yd = (double *)y;
-    timed(pi ? PROF_ROWS64 : PROF_ROWS32, eb * ent + 16.0 * R * (2.0 * N + h->t_total), 8.0 * R * ent, [&] {
+    timed(pi ? PROF_ROWS64 : PROF_ROWS32, eb * ent + eb * R * (N + h->t_total) + 32.0 * R * N, 8.0 * R * ent, [&] {
         auto launch_rows = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(ntask), dim3(64 * kRowWaves), 0, s, M.d_rtasks.as<RowTask>(),
                                g->d_leaf_start.as<int>(), M.d_lpanel.as<LeafPanel>(), M.d_psrc.as<int>(),

@@ -58,7 +58,9 @@ struct arh_server_s {
     std::vector<std::string> worker_err;
     int64_t next_id = 1;
     int live = 0;  // workers that started
+    int waiters = 0;  // callers blocked in arh_server_wait
     bool stop = false;
+    bool cancelling = false;  // set by arh_server_destroy: nothing new is admitted
     bool profiling = false;
     arh_profile prof{};
     Clock::time_point t0 = Clock::now();
@@ -154,16 +156,27 @@ static void worker(arh_server *s, int wid) {
         }
         if (rc == ARH_ENOMEM && !r->exclusive) {
             // the footprint estimate let in more than the device holds: free what the memory cache
-            // keeps and run the request again when nothing else is in flight
+            // keeps and run the request again when nothing else is in flight (unless the server is
+            // being destroyed: then it ends cancelled)
             arh_device_trim(s->opts.device, nullptr);
+            bool requeued = false;
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->used -= r->footprint;
-                r->exclusive = true;
-                r->info.status = 0;
-                s->queue.push_front(r->id);
+                if (!s->cancelling) {
+                    r->exclusive = true;
+                    r->info.status = 0;
+                    s->queue.push_front(r->id);
+                    requeued = true;
+                } else {
+                    r->err = "cancelled: the server was destroyed";
+                    r->info.t_done = s->now();
+                    r->info.status = ARH_ECANCEL;
+                }
             }
+            s->cv_done.notify_all();
             s->cv_work.notify_all();
+            (void)requeued;
             continue;
         }
         arh_profile pr{};
@@ -293,7 +306,10 @@ int arh_server_wait(arh_server *s, int64_t id, double *d, arh_request_info *info
         auto it = s->reqs.find(id);
         ARH_CHECK(it != s->reqs.end(), ARH_EINVAL, "unknown request id");
         Request *r = it->second.get();
+        s->waiters++;
         s->cv_done.wait(lk, [&] { return r->info.status >= 2 || r->info.status < 0 || s->live == 0; });
+        s->waiters--;
+        s->cv_done.notify_all();  // arh_server_destroy may be waiting for the waiters to leave
         ARH_CHECK(r->info.status != 0 && r->info.status != 1, ARH_EHIP, "no server worker could start");
         if (info) *info = r->info;
         ARH_CHECK(r->info.status == 2, r->info.status, "request failed: " + r->err);
@@ -328,22 +344,58 @@ int arh_server_get_profile(arh_server *s, arh_profile *out) {
     });
 }
 
+// Drops every queued request (status ARH_ECANCEL); requests already running finish. Caller holds s->mu.
+static int cancel_queued(arh_server *s) {
+    int n = 0;
+    for (int64_t id : s->queue) {
+        Request &r = *s->reqs[id];
+        r.err = "cancelled before it was admitted";
+        r.info.t_done = s->now();
+        r.info.status = ARH_ECANCEL;
+        n++;
+    }
+    s->queue.clear();
+    return n;
+}
+
+int arh_server_cancel(arh_server *s, int *n_cancelled) {
+    return sguard([&] {
+        ARH_CHECK(s, ARH_EINVAL, "NULL argument");
+        int n = 0;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            n = cancel_queued(s);
+        }
+        s->cv_done.notify_all();
+        if (n_cancelled) *n_cancelled = n;
+    });
+}
+
 void arh_server_destroy(arh_server *s) {
     if (!s) return;
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        // let queued work finish before stopping
+        // queued requests are cancelled; running ones finish (a kernel in flight cannot be stopped),
+        // then the workers leave and every caller still blocked in arh_server_wait has returned
+        s->cancelling = true;
+        cancel_queued(s);
+        s->cv_done.notify_all();
         s->cv_done.wait(lk, [&] {
-            if (s->live == 0) return true;
-            if (!s->queue.empty()) return false;
-            for (auto &kv : s->reqs)
-                if (kv.second->info.status == 1) return false;
-            return true;
+            if (s->live > 0)
+                for (auto &kv : s->reqs)
+                    if (kv.second->info.status == 1) return false;
+            return s->queue.empty();
         });
         s->stop = true;
     }
     s->cv_work.notify_all();
     for (auto &t : s->threads) t.join();
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->live = 0;
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+    }
     delete s;
 }
 

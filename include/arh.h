@@ -37,6 +37,7 @@ extern "C" {
 #define ARH_ECONV -4    /* solver did not reach tolerance (only when opted in) */
 #define ARH_ENOMEM -5   /* device allocation failed or over the admission budget */
 #define ARH_ESTATE -6   /* call order (e.g. not assembled) */
+#define ARH_ECANCEL -7  /* serving: the request was cancelled before it was admitted */
 
 /* Storage / arithmetic of an assembled H-matrix. */
 #define ARH_FP64 0  /* complex FP64 leaves and factors */
@@ -94,8 +95,9 @@ typedef struct {
 /* Live kernel timing (HIP events on the context's stream) of the two H-matvec kernels:
  * lowrank = the per-block t = V x (or U^T x) pass, rows = the per-leaf dense-panel + U pass (or
  * the per-column-leaf pass when transposed). bytes = algorithmic bytes of the launches: matrix
- * entries read once (16 B complex FP64, 8 B complex FP32) + right-hand sides / intermediates /
- * results once per rhs. flops = 8 per complex multiply-add. */
+ * entries read once (16 B complex FP64, 8 B complex FP32), the packed right-hand sides and
+ * intermediates once per rhs in the operator's precision (16 B / 8 B), and in the rows pass the
+ * FP64 x read and y write once per rhs (16 B each). flops = 8 per complex multiply-add. */
 typedef struct {
     int64_t launches;
     double ms;
@@ -293,6 +295,14 @@ int arh_server_poll(arh_server *s, int64_t request_id, arh_request_info *info);
  * each request admitted while enabled adds its launches when it completes. */
 int arh_server_profile(arh_server *s, int enable);
 int arh_server_get_profile(arh_server *s, arh_profile *out);
+/* Drop every queued (not yet admitted) request: each ends with status ARH_ECANCEL, and
+ * arh_server_wait on it returns ARH_ECANCEL. Requests already running finish normally.
+ * n_cancelled (may be NULL) receives how many were dropped. */
+int arh_server_cancel(arh_server *s, int *n_cancelled);
+/* Cancel what is queued (as arh_server_cancel), let the running requests finish, stop and join the
+ * worker threads, and wait until no caller is blocked in arh_server_wait; then free the server.
+ * Call it before the process tears down the HIP runtime (e.g. from an exit hook): worker threads
+ * still issuing HIP work during that teardown crash the process. */
 void arh_server_destroy(arh_server *s);
 
 #ifdef __cplusplus

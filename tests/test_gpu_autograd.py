@@ -72,3 +72,27 @@ def test_autograd_vs_oracle_and_recompute(small_relief, oracle):
     d2, _ = eng2.forward(h)
     g2 = eng2.backward(np.array([1.0, 0.0]))
     assert np.array_equal(d2, d_b) and np.allclose(g2, g, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_backward_uses_its_own_forward(small_relief, capi, recompute):
+    """Forward at h1, another forward at h2 (a line search or an FD probe), then the backward of the h1
+    output: it must be the gradient at h1, bit for bit equal to a forward + backward at h1 alone."""
+    import torch
+    from acoustic_reliefs_amd import autograd
+    eng = _engine(small_relief, capi.ARH_FP64, recompute=recompute)
+    HH = len(eng.diff)
+    h1 = torch.tensor(0.01 * np.cos(np.arange(HH) * 0.3), dtype=torch.float64, requires_grad=True)
+    h2 = torch.tensor(0.02 * np.sin(np.arange(HH) * 0.5), dtype=torch.float64, requires_grad=True)
+    d1 = autograd.diffusion_coefficients(h1, eng)
+    d2 = autograd.diffusion_coefficients(h2, eng)
+    eng.forward(h2.detach().numpy() * 0.5)  # a plain probe in between as well
+    (1.0 - d1).mean().backward()
+    g1 = h1.grad.numpy().copy()
+    (1.0 - d2).mean().backward()
+    g2 = h2.grad.numpy().copy()
+    ref = _engine(small_relief, capi.ARH_FP64, recompute=recompute)
+    hr = torch.tensor(h1.detach().numpy(), dtype=torch.float64, requires_grad=True)
+    (1.0 - autograd.diffusion_coefficients(hr, ref)).mean().backward()
+    assert np.array_equal(g1, hr.grad.numpy())
+    assert not np.allclose(g1, g2)
