@@ -109,6 +109,7 @@ SIGNATURES = [
                           ctypes.POINTER(SolveStats)]),
     ("arh_rhs", c_int, [c_void_p, c_double, c_int, c_int, P_D, c_void_p]),
     ("arh_receivers", c_int, [c_void_p, c_double, P_D, c_int, c_int, c_int, c_void_p, c_void_p]),
+    ("arh_receivers_noalloc", c_int, [c_void_p, c_double, P_D, c_int, c_int, c_int, c_void_p, c_void_p]),
     ("arh_relief_create", c_int, [c_void_p, P_D, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("arh_relief_eval", c_int, [c_void_p, c_void_p, c_void_p]),
     ("arh_relief_adjoint", c_int, [c_void_p, c_void_p, c_void_p]),
@@ -363,6 +364,12 @@ class Geometry:
         _check(lib().arh_receivers(self.h, float(k), _dp(L), LL, int(transpose), int(nrhs), _ptr(x), _ptr(out)),
                self.ctx.h)
         return out
+
+    def receivers_noalloc(self, k, L, x, y, nrhs=1, transpose=False):
+        """arh_receivers_noalloc into a caller-provided y (device): no device allocation in the library."""
+        L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 3)
+        _check(lib().arh_receivers_noalloc(self.h, float(k), _dp(L), len(L), int(transpose), int(nrhs), _ptr(x),
+                                           _ptr(y)), self.ctx.h)
 
     def listener_aca(self, k, L, tol, max_rank=5000):
         """Listener ACA Q ~= U V (compute_ACA_listener, aca.cpp:84-134) built on the device."""

@@ -355,6 +355,12 @@ int mfma_max_leaf();
 void launch_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs_dev, double2 *b);
 void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int transpose, int nrhs, const double2 *x,
                       double2 *y);
+// Receivers with the coordinates in a static per-GPU array (no device allocation); at most
+// kNoAllocMaxReceivers receivers, devices 0 .. kNoAllocMaxDevices - 1.
+constexpr int kNoAllocMaxReceivers = 16384;
+constexpr int kNoAllocMaxDevices = 64;
+void launch_receivers_noalloc(arh_geom *g, double k, const double *L_host, int LL, int transpose, int nrhs,
+                              const double2 *x, double2 *y);
 void launch_scale(hipStream_t s, double2 *a, int64_t n, double f);
 void launch_diffusion(hipStream_t s, int LL, int nrhs, const double2 *y, double *d, double *u, double2 *vf);
 // relief.hip

@@ -466,6 +466,16 @@ int arh_receivers(arh_geom *g, double k, const double *L, int LL, int transpose,
     });
 }
 
+int arh_receivers_noalloc(arh_geom *g, double k, const double *L, int LL, int transpose, int nrhs, const double *x,
+                          double *y) {
+    return guard(g ? g->ctx : nullptr, [&] {
+        ARH_CHECK(g && L && x && y, ARH_EINVAL, "NULL argument");
+        ARH_CHECK(LL >= 1 && nrhs >= 1, ARH_EINVAL, "LL and nrhs must be >= 1");
+        set_device(g->ctx);
+        launch_receivers_noalloc(g, k, L, LL, transpose, nrhs, (const double2 *)x, (double2 *)y);
+    });
+}
+
 int arh_qaca_create(arh_geom *g, double k, const double *L, int LL, double tol, int max_rank, arh_qaca **out) {
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && L && out, ARH_EINVAL, "NULL argument");

@@ -1,5 +1,7 @@
 // Right-hand sides, the receiver (listener) matrix product and the diffusion coefficient.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "arh_internal.h"
 #include "device_common.h"
@@ -50,19 +52,35 @@ __global__ void k_qmat(const Tri *__restrict__ tri, int N, double k, const doubl
 constexpr int kRT = 256;
 constexpr int kRC = 8;
 
+// Sources of the receiver-matrix entries Q_l,i: read from a materialised Q (LL x N, row-major) or
+// computed on the fly from the receiver coordinates (the same bem_entry k_qmat evaluates).
+struct QStored {
+    const double2 *__restrict__ Q;
+    int N;
+    __device__ double2 operator()(int l, int i) const { return Q[(int64_t)l * N + i]; }
+};
+struct QOnTheFly {
+    const Tri *__restrict__ tri;
+    const double *__restrict__ L;
+    double k;
+    __device__ double2 operator()(int l, int i) const {
+        return bem_entry(k, L[3 * l], L[3 * l + 1], L[3 * l + 2], tri[i]);
+    }
+};
+
 // y[j, l] = sum_i Q[l, i] x[j, i]; one workgroup per receiver, rhs in chunks of kRC.
-__global__ __launch_bounds__(kRT) void k_qx(const double2 *__restrict__ Q, int N, int LL, int R,
-                                            const double2 *__restrict__ x, double2 *__restrict__ y) {
+template <class Src>
+__global__ __launch_bounds__(kRT) void k_qx(Src q, int N, int LL, int R, const double2 *__restrict__ x,
+                                            double2 *__restrict__ y) {
     __shared__ double2 sh[kRT / ARH_WAVE][kRC];
     const int l = blockIdx.x;
     const int lane = threadIdx.x % ARH_WAVE, w = threadIdx.x / ARH_WAVE;
-    const double2 *q = Q + (int64_t)l * N;
     for (int j0 = 0; j0 < R; j0 += kRC) {
         double2 acc[kRC];
 #pragma unroll
         for (int c = 0; c < kRC; c++) acc[c] = make_double2(0.0, 0.0);
         for (int i = threadIdx.x; i < N; i += kRT) {
-            double2 a = q[i];
+            double2 a = q(l, i);
 #pragma unroll
             for (int c = 0; c < kRC; c++)
                 if (j0 + c < R) cfma(acc[c], a, x[(int64_t)(j0 + c) * N + i]);
@@ -83,13 +101,14 @@ __global__ __launch_bounds__(kRT) void k_qx(const double2 *__restrict__ Q, int N
 }
 
 // y[j, i] = sum_l Q[l, i] x[j, l] (the adjoint right-hand side Q^T vf, adjoint.cpp:198).
-__global__ __launch_bounds__(kRT) void k_qtx(const double2 *__restrict__ Q, int N, int LL, int R,
-                                             const double2 *__restrict__ x, double2 *__restrict__ y) {
+template <class Src>
+__global__ __launch_bounds__(kRT) void k_qtx(Src q, int N, int LL, int R, const double2 *__restrict__ x,
+                                             double2 *__restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= N) return;
     double2 acc = make_double2(0.0, 0.0);
-    for (int l = 0; l < LL; l++) cfma(acc, Q[(int64_t)l * N + i], x[(int64_t)j * LL + l]);
+    for (int l = 0; l < LL; l++) cfma(acc, q(l, i), x[(int64_t)j * LL + l]);
     y[(int64_t)j * N + i] = acc;
 }
 
@@ -102,12 +121,59 @@ void launch_receivers(arh_geom *g, double k, const double *L_dev, int LL, int tr
     Q.alloc(sizeof(double2) * (size_t)LL * N);
     hipLaunchKernelGGL(k_qmat, dim3((N + 255) / 256, LL), dim3(256), 0, s, g->tri.as<Tri>(), N, k, L_dev,
                        Q.as<double2>());
+    const QStored src{Q.as<double2>(), N};
     if (!transpose)
-        hipLaunchKernelGGL(k_qx, dim3(LL), dim3(kRT), 0, s, Q.as<double2>(), N, LL, nrhs, x, y);
+        hipLaunchKernelGGL(k_qx<QStored>, dim3(LL), dim3(kRT), 0, s, src, N, LL, nrhs, x, y);
     else
-        hipLaunchKernelGGL(k_qtx, dim3((N + kRT - 1) / kRT, nrhs), dim3(kRT), 0, s, Q.as<double2>(), N, LL, nrhs, x, y);
+        hipLaunchKernelGGL(k_qtx<QStored>, dim3((N + kRT - 1) / kRT, nrhs), dim3(kRT), 0, s, src, N, LL, nrhs, x, y);
     ARH_HIP(hipGetLastError());
     ARH_HIP(hipStreamSynchronize(s));  // Q is freed on return
+}
+
+// ---------------------------------------------------------------- receivers without allocation
+// arh_receivers_noalloc keeps the receiver coordinates in this file-scope device array (one copy
+// per GPU, held until the process ends) and evaluates the receiver-matrix entries on the fly, so it
+// needs no device memory beyond its inputs and outputs. Every call on a GPU shares that GPU's
+// array: calls that overlap on one GPU (other streams, other threads) race.
+static __device__ __align__(256) double scratch[3 * kNoAllocMaxReceivers];
+
+namespace {
+std::mutex scratch_mu;
+double *scratch_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+// The device address of this GPU's copy of scratch (the array's name is only a host-side stand-in);
+// looked up once per device, with that device current.
+static double *scratch_on(int device) {
+    ARH_CHECK(device >= 0 && device < kNoAllocMaxDevices, ARH_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void *p = nullptr;
+        ARH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ARH_CHECK(p != nullptr, ARH_EHIP, "hipGetSymbolAddress returned no address for the receiver scratch");
+        scratch_ptr[device] = static_cast<double *>(p);
+    }
+    return scratch_ptr[device];
+}
+
+void launch_receivers_noalloc(arh_geom *g, double k, const double *L_host, int LL, int transpose, int nrhs,
+                              const double2 *x, double2 *y) {
+    ARH_CHECK(LL <= kNoAllocMaxReceivers, ARH_EINVAL,
+              "arh_receivers_noalloc takes at most " + std::to_string(kNoAllocMaxReceivers) + " receivers, got " +
+                  std::to_string(LL));
+    arh_ctx *ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    double *Ld = scratch_on(ctx->device);
+    // stream-ordered after earlier work on this stream that may still read the previous coordinates
+    ARH_HIP(hipMemcpyAsync(Ld, L_host, sizeof(double) * 3 * (size_t)LL, hipMemcpyHostToDevice, s));
+    const QOnTheFly src{g->tri.as<Tri>(), Ld, k};
+    const int N = g->N;
+    if (!transpose)
+        hipLaunchKernelGGL(k_qx<QOnTheFly>, dim3(LL), dim3(kRT), 0, s, src, N, LL, nrhs, x, y);
+    else
+        hipLaunchKernelGGL(k_qtx<QOnTheFly>, dim3((N + kRT - 1) / kRT, nrhs), dim3(kRT), 0, s, src, N, LL, nrhs, x,
+                           y);
+    ARH_HIP(hipGetLastError());
 }
 
 // Diffusion coefficient per rhs with n_freqs = 1 (adjoint.cpp:112-127, 145, 388-397).

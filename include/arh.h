@@ -203,6 +203,16 @@ int arh_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs, doubl
 int arh_receivers(arh_geom *g, double k, const double *L, int LL, int transpose, int nrhs, const double *x,
                   double *y);
 
+/* No counterpart in the reference: arh_receivers for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as arh_receivers; the entries Q_lj are evaluated on the fly and the receiver
+ * coordinates are staged in a static per-GPU array of 16384 receivers (384 KiB, held until the
+ * process ends), so at most LL = 16384 (ARH_EINVAL above). No hipMalloc / hipFree; work is queued on
+ * ctx's stream and the call returns without waiting for it. Calls that overlap on one GPU (other
+ * streams or threads) share the array and race. */
+int arh_receivers_noalloc(arh_geom *g, double k, const double *L, int LL, int transpose, int nrhs, const double *x,
+                          double *y);
+
 /* Listener ACA (compute_ACA_listener, aca.cpp:84-134): partially pivoted cross approximation
  * Q ~= U V of the whole LL x N listener matrix Q_lj = M(L_l, element j), started at row 0, rows
  * divided by the pivot without a zero guard, stopped when sqrt(|u|^2 |v|^2 / err) <= tol (the
