@@ -176,7 +176,7 @@ def measured_traffic(kernel="k_rows_mfma<float, 5, 2>"):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 4 and cells[0] == "`%s`" % kernel:
+        if len(cells) >= 4 and cells[0] == "`%s`" % kernel:
             try:
                 return float(cells[3]) * 1e6
             except ValueError:
@@ -221,6 +221,54 @@ def isolated_roofline(g, capi, ctx, f=4000.0, reps=20):
                               % rw["launches"]}
     del h
     return out
+
+
+# Declared FP64 flops per kernel evaluation, used for the FP64-VALU rooflines of the assembly and the shape
+# gradient (transcendental functions, square roots and divisions counted as one flop each, an FMA as two):
+# * one quadrature point of an entry (the 6-point rule of integrate.cpp:5-14): rule point 12, distance 11,
+#   s = -(d.n) 5, k r 1, sincos 1, w s / (r^2 r) 3, the two accumulated parts 8: 41 (+ 3 for the sqrt, the
+#   division and the rule-point select) = 44;
+# * one quadrature point of a shape derivative (dM_ie/dp_i or dM_ei/dq_(i,c), adjoint.cpp:244-352): rule point
+#   12, distance 12, s 5, F and F' 31, the collocation term 12 or the three corner terms 48: 90 on average.
+FLOPS_PER_ENTRY_POINT = 44
+FLOPS_PER_DERIV_POINT = 90
+
+
+def kernel_counts(g, capi, ctx, top, f=4000.0, R=37):
+    """Algorithmic work of the FP64-VALU kernels on the f Hz operator: the ACA kernel (entry evaluations of the
+    crosses it computes: rank + 1 crosses per block, the last one discarded, or the rank when the limit
+    min(100, m - 1, n - 1) stops it) and the dense-leaf kernel, then one shape-gradient call with R right-hand
+    sides (12 derivative points per (touched element, element) pair and pass, plus 8 flops per complex
+    multiply-add of the R right-hand sides). Run under rocprofv3 --kernel-trace for the kernel durations."""
+    k = 2 * np.pi * f / C0
+    h = g.assemble(k, 1e-4, 100, capi.ARH_FP64)
+    ranks = h.ranks()
+    m = g.lowrank[:, 1] - g.lowrank[:, 0]
+    n = g.lowrank[:, 3] - g.lowrank[:, 2]
+    lim = np.minimum(100, np.minimum(m - 1, n - 1))
+    crosses = np.where(ranks < lim, ranks + 1, ranks)
+    aca_entries = int(np.sum(crosses * (m + n)))
+    dense_entries = int(np.sum((g.dense[:, 1] - g.dense[:, 0]) * (g.dense[:, 3] - g.dense[:, 2])))
+    del h
+    P, E = g.P, g.E
+    top = np.asarray(top, np.int32)
+    touched = np.any(np.isin(E, top), axis=1).sum()
+    rng = np.random.default_rng(0)
+    X = ctx.to_device(rng.standard_normal((R, g.N)) + 1j * rng.standard_normal((R, g.N)))
+    LAM = ctx.to_device(rng.standard_normal((R, g.N)) + 1j * rng.standard_normal((R, g.N)))
+    L = np.column_stack([5 * np.cos(np.linspace(0, np.pi, 181)), 5 * np.sin(np.linspace(0, np.pi, 181)), np.zeros(181)])
+    VF = ctx.to_device(rng.standard_normal((R, 181)) + 1j * rng.standard_normal((R, 181)))
+    dirs = -np.column_stack([np.cos(np.linspace(0, np.pi, R)), np.sin(np.linspace(0, np.pi, R)), np.zeros(R)])
+    g.shape_grad(k, L, top, R, X, LAM, VF, capi.ARH_RHS_PLANE, dirs)
+    ctx.synchronize()
+    passes = -(-R // 20)
+    pairs = int(touched) * g.N
+    return {"frequency": f, "N": g.N, "aca_entries": aca_entries, "aca_points": 6 * aca_entries,
+            "aca_flops": 6 * aca_entries * FLOPS_PER_ENTRY_POINT, "dense_entries": dense_entries,
+            "dense_flops": 6 * dense_entries * FLOPS_PER_ENTRY_POINT, "grad_touched": int(touched),
+            "grad_pairs_per_pass": pairs, "grad_passes": passes,
+            "grad_B_flops": passes * pairs * 12 * FLOPS_PER_DERIV_POINT + pairs * 4 * 8 * R,
+            "flops_per_entry_point": FLOPS_PER_ENTRY_POINT, "flops_per_deriv_point": FLOPS_PER_DERIV_POINT}
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -572,6 +620,8 @@ def main():
     ap.add_argument("--leaf", type=int, default=128, help="with --diffbem: cluster_size")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated timing of the dominant kernel (the roofline figures)")
+    ap.add_argument("--kernel-counts", action="store_true",
+                    help="only the ACA assembly and one shape-gradient call at 4000 Hz, printing their algorithmic work")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -586,6 +636,13 @@ def main():
         return
     if args.stream:
         run_stream(args)
+        return
+    if args.kernel_counts:
+        from acoustic_reliefs_amd import capi, meshes
+        P, E, top = meshes.relief_config(size=0.6, grid=args.grid, hmax=0.1, seed=0)
+        ctx = capi.Context(0)
+        g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
+        print(json.dumps(kernel_counts(g, capi, ctx, top)), flush=True)
         return
     if args.roofline_only:
         from acoustic_reliefs_amd import capi
