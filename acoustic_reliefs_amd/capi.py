@@ -92,10 +92,12 @@ SIGNATURES = [
     ("arh_free", c_int, [c_void_p, c_void_p]),
     ("arh_upload", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     ("arh_download", c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    ("arh_add", c_int, [c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
     ("arh_partition", c_int, [P_D, c_int, c_int, c_double, P_I, P_I, c_int, P_I, P_I]),
     ("arh_geom_create", c_int, [c_void_p, P_D, c_int, P_I, c_int, P_I, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("arh_geom_set_vertices", c_int, [c_void_p, P_D]),
     ("arh_geom_set_vertices_device", c_int, [c_void_p, c_void_p]),
+    ("arh_geom_set_heights_device", c_int, [c_void_p, c_void_p, P_I, c_int, c_void_p]),
     ("arh_geom_destroy", None, [c_void_p]),
     ("arh_hmat_assemble", c_int, [c_void_p, c_double, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("arh_hmat_get_info", c_int, [c_void_p, ctypes.POINTER(HMatInfo)]),
@@ -122,6 +124,8 @@ SIGNATURES = [
     ("arh_diffusion", c_int, [c_void_p, c_int, c_int, c_void_p, P_D, c_void_p, c_void_p]),
     ("arh_shape_grad", c_int, [c_void_p, c_double, P_D, c_int, P_I, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
                                P_D, c_void_p, c_void_p, c_void_p]),
+    ("arh_shape_grad_accumulate", c_int, [c_void_p, c_double, P_D, c_int, P_I, c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_int, P_D, c_void_p]),
     ("arh_band_solve", c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, P_D, P_D, c_int,
                                ctypes.POINTER(SolveOpts), P_D, c_void_p, c_void_p, ctypes.POINTER(SolveStats),
                                ctypes.POINTER(HMatInfo)]),

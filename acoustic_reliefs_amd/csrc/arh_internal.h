@@ -332,6 +332,7 @@ struct arh_relief_s {
 
 namespace arh {
 // geometry.hip
+void geom_set_heights(arh_geom *g, const double *base, const int *vid_dev, int HH, const double *h);
 void geom_build_tri(arh_geom *g);
 // assemble.hip
 void assemble_dense(arh_hmat *h);
@@ -362,6 +363,7 @@ constexpr int kNoAllocMaxDevices = 64;
 void launch_receivers_noalloc(arh_geom *g, double k, const double *L_host, int LL, int transpose, int nrhs,
                               const double2 *x, double2 *y);
 void launch_scale(hipStream_t s, double2 *a, int64_t n, double f);
+void launch_add(hipStream_t s, const double *x, double *y, int64_t n);
 void launch_diffusion(hipStream_t s, int LL, int nrhs, const double2 *y, double *d, double *u, double2 *vf);
 // relief.hip
 void relief_build(arh_relief *r, const double *uvs);
@@ -376,5 +378,5 @@ void solve(arh_hmat *h, int transpose, int nrhs, const double2 *b, double2 *x, c
 // grad.hip
 void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const int *diff, int HH, int nrhs,
                        const double2 *x, const double2 *lam, const double2 *vf, int src_kind, const double *src_dev,
-                       double *A, double *B, double *C);
+                       double *A, double *B, double *C, double *G = nullptr);
 }  // namespace arh

@@ -203,6 +203,14 @@ int arh_download(arh_ctx *ctx, void *dst, const void *src, size_t bytes) {
     });
 }
 
+int arh_add(arh_ctx *ctx, int64_t n, const double *x, double *y) {
+    return guard(ctx, [&] {
+        ARH_CHECK(ctx && (n == 0 || (x && y)) && n >= 0, ARH_EINVAL, "NULL argument or negative length");
+        set_device(ctx);
+        launch_add(ctx->stream, x, y, n);
+    });
+}
+
 int arh_partition(const double *C, int N, int leaf_size, double radius_factor, int *perm, int *blocks, int cap,
                   int *n_dense, int *n_lowrank) {
     return guard(nullptr, [&] {
@@ -263,6 +271,24 @@ int arh_geom_set_vertices(arh_geom *g, const double *Ps) {
         ARH_HIP(hipMemcpyAsync(g->P.p, Ps, sizeof(double) * 3 * g->V, hipMemcpyHostToDevice, g->ctx->stream));
         geom_build_tri(g);
         ARH_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int arh_geom_set_heights_device(arh_geom *g, const double *base, const int *vids, int HH, const double *h) {
+    return guard(g ? g->ctx : nullptr, [&] {
+        ARH_CHECK(g && base && (HH == 0 || (vids && h)), ARH_EINVAL, "NULL argument");
+        ARH_CHECK(HH >= 0, ARH_EINVAL, "HH must be >= 0");
+        std::vector<char> seen(g->V, 0);
+        for (int t = 0; t < HH; t++) {
+            if (vids[t] < 0) continue;  // a skipped entry
+            ARH_CHECK(vids[t] < g->V, ARH_EINVAL, "vertex id out of range");
+            ARH_CHECK(!seen[vids[t]], ARH_EINVAL, "repeated vertex id");
+            seen[vids[t]] = 1;
+        }
+        set_device(g->ctx);
+        DBuf vd;
+        upload(vd, std::vector<int>(vids, vids + HH), g->ctx->stream);
+        geom_set_heights(g, base, vd.as<int>(), HH, h);
     });
 }
 
@@ -598,6 +624,21 @@ int arh_shape_grad(arh_geom *g, double k, const double *L, int LL, const int *di
         upload(Sd, std::vector<double>(src, src + 3 * (size_t)nrhs), g->ctx->stream);
         launch_shape_grad(g, k, Ld.as<double>(), LL, diff, HH, nrhs, (const double2 *)x, (const double2 *)lam,
                           (const double2 *)vf, src_kind, Sd.as<double>(), A, B, C);
+    });
+}
+
+int arh_shape_grad_accumulate(arh_geom *g, double k, const double *L, int LL, const int *diff, int HH, int nrhs,
+                              const double *x, const double *lam, const double *vf, int src_kind, const double *src,
+                              double *grad) {
+    return guard(g ? g->ctx : nullptr, [&] {
+        ARH_CHECK(g && L && diff && x && lam && vf && src && grad, ARH_EINVAL, "NULL argument");
+        ARH_CHECK(HH >= 0 && nrhs >= 1 && LL >= 1, ARH_EINVAL, "bad sizes");
+        set_device(g->ctx);
+        DBuf Ld, Sd;
+        upload(Ld, std::vector<double>(L, L + 3 * (size_t)LL), g->ctx->stream);
+        upload(Sd, std::vector<double>(src, src + 3 * (size_t)nrhs), g->ctx->stream);
+        launch_shape_grad(g, k, Ld.as<double>(), LL, diff, HH, nrhs, (const double2 *)x, (const double2 *)lam,
+                          (const double2 *)vf, src_kind, Sd.as<double>(), nullptr, nullptr, nullptr, grad);
     });
 }
 

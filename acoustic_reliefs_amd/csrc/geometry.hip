@@ -29,6 +29,25 @@ __global__ void k_tri(const double *__restrict__ P, const int *__restrict__ E, i
     out[e] = t;
 }
 
+// Vertex vid[t] (>= 0) gets y = h[t] + base y (the y offset of DiffBEM's differentiable points,
+// acoustics3d.cpp:139-142); the other coordinates are the base ones (copied before the launch).
+__global__ void k_offset_y(const double *__restrict__ base, const int *__restrict__ vid, int HH,
+                           const double *__restrict__ h, double *__restrict__ P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= HH) return;
+    const int v = vid[t];
+    if (v >= 0) P[3 * v + 1] = h[t] + base[3 * v + 1];
+}
+
+void geom_set_heights(arh_geom *g, const double *base, const int *vid_dev, int HH, const double *h) {
+    hipStream_t s = g->ctx->stream;
+    ARH_HIP(hipMemcpyAsync(g->P.p, base, sizeof(double) * 3 * g->V, hipMemcpyDeviceToDevice, s));
+    if (HH > 0)
+        hipLaunchKernelGGL(k_offset_y, dim3((HH + 255) / 256), dim3(256), 0, s, base, vid_dev, HH, h, g->P.as<double>());
+    ARH_HIP(hipGetLastError());
+    geom_build_tri(g);
+}
+
 void geom_build_tri(arh_geom *g) {
     g->tri.alloc(sizeof(Tri) * (size_t)g->N);
     hipLaunchKernelGGL(k_tri, dim3((g->N + 255) / 256), dim3(256), 0, g->ctx->stream, g->P.as<double>(),

@@ -272,11 +272,12 @@ __global__ __launch_bounds__(kGT) void k_grad_AC(const Tri *__restrict__ tri, in
         for (int c = 0; c < 3; c++) cA[t * 3 + c] = a[c], cC[t * 3 + c] = cc;
 }
 
-// gradient_h: sum over (touched element, corner) in list order; B summed over the splits first.
+// gradient_h: sum over (touched element, corner) in list order; B summed over the splits first. With G, the
+// terms are not stored but added: G_h = G_h + ((A_h - B_h) + C_h), the order of the host sum A - B + C.
 __global__ void k_grad_scatter(int HH, int T, int esplit, const int *__restrict__ hptr, const int *__restrict__ hlist,
                                const double *__restrict__ cA, const double *__restrict__ partB,
                                const double *__restrict__ cC, double *__restrict__ A, double *__restrict__ B,
-                               double *__restrict__ C) {
+                               double *__restrict__ C, double *__restrict__ G) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= HH) return;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -288,12 +289,15 @@ __global__ void k_grad_scatter(int HH, int T, int esplit, const int *__restrict_
         b += bb;
         c += cC[tc];
     }
-    A[h] = a, B[h] = b, C[h] = c;
+    if (G)
+        G[h] = G[h] + ((a - b) + c);
+    else
+        A[h] = a, B[h] = b, C[h] = c;
 }
 
 void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const int *diff, int HH, int nrhs,
                        const double2 *x, const double2 *lam, const double2 *vf, int src_kind, const double *src_dev,
-                       double *A, double *B, double *C) {
+                       double *A, double *B, double *C, double *G) {
     hipStream_t s = g->ctx->stream;
     const int N = g->N, V = g->V;
     // slots of differentiable vertices; touched elements; per-h (element, corner) lists
@@ -320,6 +324,7 @@ void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const
         hlist.insert(hlist.end(), lists[h].begin(), lists[h].end());
         hptr[h + 1] = (int)hlist.size();
     }
+    if (T == 0 && G) return;  // nothing to add
     if (T == 0) {
         ARH_HIP(hipMemsetAsync(A, 0, sizeof(double) * HH, s));
         ARH_HIP(hipMemsetAsync(B, 0, sizeof(double) * HH, s));
@@ -347,7 +352,7 @@ void launch_shape_grad(arh_geom *g, double k, const double *L_dev, int LL, const
     hipLaunchKernelGGL(k_grad_AC, dim3(gxb), dim3(kGT), 0, s, g->tri.as<Tri>(), N, T, d_touched.as<int>(), k, nrhs, x,
                        lam, vf, L_dev, LL, src_kind, src_dev, cA.as<double>(), cC.as<double>());
     hipLaunchKernelGGL(k_grad_scatter, dim3((HH + 255) / 256), dim3(256), 0, s, HH, T, esplit, d_hptr.as<int>(),
-                       d_hlist.as<int>(), cA.as<double>(), pB.as<double>(), cC.as<double>(), A, B, C);
+                       d_hlist.as<int>(), cA.as<double>(), pB.as<double>(), cC.as<double>(), A, B, C, G);
     ARH_HIP(hipGetLastError());
     ARH_HIP(hipStreamSynchronize(s));
 }

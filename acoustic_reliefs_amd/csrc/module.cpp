@@ -199,6 +199,7 @@ public:
         std::vector<int> E(Es.data(), Es.data() + Es.size());
         {
             py::gil_scoped_release nogil;
+            base_dev_.reset();
             mesh_ = make_mesh(ctx_, std::move(P), E, cluster_size_, radius_factor_, true);
             dense_mesh_.reset();
         }
@@ -287,6 +288,35 @@ public:
         py::array_t<double> out(HH_);
         std::copy(g.begin(), g.end(), out.mutable_data());
         return py::make_tuple(c, out);
+    }
+
+    // gradient(x) for heights and a gradient that live on the device (a torch optimiser's tensors): x_ptr and
+    // grad_ptr are device addresses of HH float64 values, read and written on the library's stream; the call
+    // returns once grad is written. Same arithmetic, in the same order, as gradient(x): the heights offset the
+    // base mesh on the device, each band's sum over its frequencies is formed on the device and added to grad.
+    double gradient_device(uintptr_t x_ptr, uintptr_t grad_ptr) {
+        if (!mesh_set_) {
+            std::cerr << "Error! No mesh set. Set one using set_mesh or precompute." << std::endl;
+            return 0.0;
+        }
+        if (!diff_set_) {
+            std::cerr << "Error! No differentiable points set. Set them using set_diff_pts or precompute." << std::endl;
+            return -1.0;
+        }
+        if (!x_ptr || !grad_ptr) throw std::invalid_argument("x_ptr and grad_ptr must be device addresses");
+        double c = 0;
+        const double *xd = reinterpret_cast<const double *>(x_ptr);
+        double *gd = reinterpret_cast<double *>(grad_ptr);
+        std::vector<double> zeros(std::max(1, HH_), 0.0);
+        DevBuf band_grad(ctx_->c, sizeof(double) * std::max(1, HH_));
+        if (HH_ > 0) check(arh_upload(ctx_->c, gd, zeros.data(), sizeof(double) * HH_), ctx_->c);
+        for (double F : freq_bands_) {  // band() releases the GIL around its GPU work
+            if (HH_ > 0) band_grad.up(zeros.data(), sizeof(double) * HH_);
+            c += band(mesh_->P, F, false, nullptr, xd, band_grad.d());
+            if (HH_ > 0) check(arh_add(ctx_->c, HH_, band_grad.d(), gd), ctx_->c);
+        }
+        check(arh_ctx_synchronize(ctx_->c), ctx_->c);
+        return c;
     }
 
     py::array_t<cd> pvals1(double f) { return pvals2(f, f64arr(py::array_t<double>({(ssize_t)L_.size() / 3, (ssize_t)3}, L_.data()))); }
@@ -383,6 +413,13 @@ private:
     // The mesh the solves run on: the H-matrix partition, or (use_actual) every leaf pair dense,
     // the stand-in for the reference's dense matrix + colPivHouseholderQr (adjoint.cpp:68-83).
     Mesh &active_mesh(const std::vector<double> &P) {
+        Mesh &m = select_mesh();
+        check(arh_geom_set_vertices(m.g, P.data()), ctx_->c);
+        return m;
+    }
+
+    // The H-matrix partition's mesh, or (use_actual) the all-dense one, built on first use.
+    Mesh &select_mesh() {
         Mesh *m = mesh_.get();
         if (use_actual) {
             if (!dense_mesh_) {
@@ -404,8 +441,19 @@ private:
             }
             m = dense_mesh_.get();
         }
-        check(arh_geom_set_vertices(m->g, P.data()), ctx_->c);
         return *m;
+    }
+
+    // active_mesh with the heights on the device: the base mesh (a device copy of the set_mesh vertices, made
+    // once) with y of each differentiable point offset by x[slot] (acoustics3d.cpp:210-213).
+    Mesh &active_mesh_device(const double *x_dev) {
+        Mesh &m = select_mesh();
+        if (!base_dev_) {
+            base_dev_ = std::make_unique<DevBuf>(ctx_->c, sizeof(double) * mesh_->P.size());
+            base_dev_->up(mesh_->P.data(), sizeof(double) * mesh_->P.size());
+        }
+        check(arh_geom_set_heights_device(m.g, base_dev_->d(), diff_.data(), HH_, x_dev), ctx_->c);
+        return m;
     }
 
     // The H-matrix mesh whatever use_actual says (pvals and surface_vals always take it,
@@ -461,9 +509,12 @@ private:
     }
 
     // bem3d::gradient for one band (adjoint.cpp:24-242) on vertex positions P.
-    double band(const std::vector<double> &P, double F, bool forward_only, std::vector<double> *grad) {
+    // x_dev (device heights) replaces P when given; grad_dev (device, HH) receives the band's gradient added
+    // frequency by frequency instead of grad.
+    double band(const std::vector<double> &P, double F, bool forward_only, std::vector<double> *grad,
+                const double *x_dev = nullptr, double *grad_dev = nullptr) {
         py::gil_scoped_release nogil;
-        Mesh &m = active_mesh(P);
+        Mesh &m = x_dev ? active_mesh_device(x_dev) : active_mesh(P);
         const int N = m.N(), LL = (int)L_.size() / 3;
         auto freqs = get_frequencies(F, n_freqs_);
         std::vector<Forward> fw;
@@ -507,7 +558,7 @@ private:
         std::vector<double> u;
         const double c = diffusion(yt, u);
         if (forward_only) return c;
-        grad->assign(HH_, 0.0);
+        if (!grad_dev) grad->assign(HH_, 0.0);
         DevBuf vfd(ctx_->c, sizeof(cd) * LL), adj(ctx_->c, sizeof(cd) * N), lam(ctx_->c, sizeof(cd) * N);
         DevBuf A(ctx_->c, sizeof(double) * std::max(1, HH_)), B(ctx_->c, sizeof(double) * std::max(1, HH_)),
             C(ctx_->c, sizeof(double) * std::max(1, HH_));
@@ -538,7 +589,11 @@ private:
             if (!silent)
                 std::cout << ss.matvecs << " matrix mults, " << ss.iters << " iters, " << ss.max_err << " error"
                           << std::endl;
-            if (HH_ > 0) {
+            if (HH_ > 0 && grad_dev) {
+                check(arh_shape_grad_accumulate(m.g, st.k, L_.data(), LL, diff_.data(), HH_, 1, st.x->d(), lam.d(),
+                                                vfd.d(), ARH_RHS_POINT, src_.data(), grad_dev),
+                      ctx_->c);
+            } else if (HH_ > 0) {
                 check(arh_shape_grad(m.g, st.k, L_.data(), LL, diff_.data(), HH_, 1, st.x->d(), lam.d(), vfd.d(),
                                      ARH_RHS_POINT, src_.data(), A.d(), B.d(), C.d()),
                       ctx_->c);
@@ -563,6 +618,7 @@ private:
     int device_;
     std::shared_ptr<Ctx> ctx_;
     std::unique_ptr<Mesh> mesh_, dense_mesh_;
+    std::unique_ptr<DevBuf> base_dev_;  // device copy of mesh_->P (gradient_device)
     std::vector<double> L_;
     std::unordered_map<int, int> Hs_;
     std::vector<int> diff_;
@@ -659,6 +715,7 @@ PYBIND11_MODULE(acoustics3d_impl, m) {
         .def("band_value", &DiffBEM::band_value, py::arg("freq_band"))
         .def("values", &DiffBEM::values)
         .def("gradient", &DiffBEM::gradient, py::arg("x"))
+        .def("gradient_device", &DiffBEM::gradient_device, py::arg("x_ptr"), py::arg("grad_ptr"))
         .def("pvals", &DiffBEM::pvals1, py::arg("frequency"))
         .def("pvals", &DiffBEM::pvals2, py::arg("frequency"), py::arg("listeners"))
         .def("get_listeners", &DiffBEM::get_listeners)

@@ -214,6 +214,17 @@ __global__ void k_scale(double2 *__restrict__ a, int64_t n, double f) {
         a[i] = make_double2(f * a[i].x, f * a[i].y);
 }
 
+__global__ void k_add(const double *__restrict__ x, double *__restrict__ y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = y[i] + x[i];
+}
+
+void launch_add(hipStream_t s, const double *x, double *y, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, s, x, y, n);
+    ARH_HIP(hipGetLastError());
+}
+
 void launch_scale(hipStream_t s, double2 *a, int64_t n, double f) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_scale, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, s, a, n, f);

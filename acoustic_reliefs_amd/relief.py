@@ -69,11 +69,20 @@ def sample_heights(tex, sampler):
     return SampleHeights.apply(tex, sampler)
 
 
-def acoustic_gradient(tex, diffbem, sampler):
-    """acoustics_opt.py:37-60 with the device sampler: returns (1 - value, -dvalue/dtex)."""
-    heights = sampler.heights(tex).cpu().numpy()
-    val, grad = diffbem.gradient(heights)
-    g = torch.from_numpy(np.ascontiguousarray(grad, dtype=np.float64)).to(sampler.device)
+def acoustic_gradient(tex, diffbem, sampler, device=True):
+    """acoustics_opt.py:37-60 with the device sampler: returns (1 - value, -dvalue/dtex).
+
+    device=True (and a DiffBEM with gradient_device, as this package's module has): the heights and the vertex
+    gradient stay on the GPU (DiffBEM.gradient_device); otherwise they go through host numpy arrays as in the
+    reference (acoustics_opt.py:264). Both routes give the same bits."""
+    heights = sampler.heights(tex)
+    if device and hasattr(diffbem, "gradient_device"):
+        g = torch.empty_like(heights)
+        torch.cuda.current_stream(sampler.device).synchronize()  # g's memory is free for the library's stream
+        val = diffbem.gradient_device(heights.data_ptr(), g.data_ptr())
+    else:
+        val, grad = diffbem.gradient(heights.cpu().numpy())
+        g = torch.from_numpy(np.ascontiguousarray(grad, dtype=np.float64)).to(sampler.device)
     return 1 - val, -sampler.tex_gradient(g)
 
 
@@ -108,13 +117,14 @@ def normalize_gradients(grad, edge=0):
 
 
 def optimise(diffbem, sampler, iters, lr=1e-3, init_val=0.0, edge_border=2, weights=None, vmax=None,
-             normalize_grads=True, band_schedule=None, callback=None):
+             normalize_grads=True, band_schedule=None, callback=None, device=True):
     """The acoustics-only optimiser (acoustics_opt.py:238-347): per iteration, the padded heightfield's
     acoustic value and gradient, the smoothness and negativity (and, with vmax, barrier) terms, each
     gradient normalised and weighted, summed into the variable's gradient, one Adam step.
 
     sampler's texture shape is the padded heightfield (hfield_res x hfield_res); the variable is its
     interior. band_schedule (optional): a band per iteration passed to diffbem.set_band (sample_freq).
+    device: the acoustic term's heights and gradient stay on the GPU (acoustic_gradient).
     Returns the final padded heightfield (numpy) and the per-iteration values {term: array}."""
     w = {"ac_wt": 5.0, "sm_wt": 10.0, "ng_wt": 0.5, "ba_wt": 0.5}
     w.update(weights or {})
@@ -131,7 +141,7 @@ def optimise(diffbem, sampler, iters, lr=1e-3, init_val=0.0, edge_border=2, weig
         if band_schedule is not None:
             diffbem.set_band(float(band_schedule[it]))
         grads = {}
-        ac_v, ac_g = acoustic_gradient(full.detach(), diffbem, sampler)
+        ac_v, ac_g = acoustic_gradient(full.detach(), diffbem, sampler, device=device)
         hist["ac"][it], grads["ac"] = ac_v, ac_g[crop]
         for name, fn in (("sm", smoothness), ("ng", neg_relu)) + ((("ba", lambda t: barrier_loss(t, vmax)),)
                                                                   if vmax else ()):

@@ -135,6 +135,9 @@ int arh_malloc(arh_ctx *ctx, size_t bytes, void **dptr);
 int arh_free(arh_ctx *ctx, void *dptr);
 int arh_upload(arh_ctx *ctx, void *dst_device, const void *src_host, size_t bytes);
 int arh_download(arh_ctx *ctx, void *dst_host, const void *src_device, size_t bytes);
+/* y[i] = y[i] + x[i], i < n (device float64), queued on ctx's stream: sums of device gradients kept in their
+ * accumulation order (DiffBEM.gradient adds one band's sum over frequencies at a time, acoustics3d.cpp:224-227). */
+int arh_add(arh_ctx *ctx, int64_t n, const double *x, double *y);
 
 /* ------------------------------------------------------------------ set-up (host, once per mesh) */
 /* Cluster tree + dual-tree block partition. Replaces ClusterTree(max_pts, Cs) +
@@ -153,6 +156,11 @@ int arh_geom_create(arh_ctx *ctx, const double *Ps, int V, const int *Es, int N,
 /* Move vertices (heights applied by the caller), keeping the partition (acoustics3d.cpp:139-146, 210-217). */
 int arh_geom_set_vertices(arh_geom *g, const double *Ps_host);
 int arh_geom_set_vertices_device(arh_geom *g, const double *Ps_device);
+/* The vertices of a deformed mesh on the device (DiffBEM's differentiable points moved in y,
+ * acoustics3d.cpp:139-142 and 210-213): Ps = base (device, V x 3) with y of vertex vids[t] (host; distinct ids,
+ * or -1 to skip entry t) set to h[t] + base y (h device, HH). Queued on the context's stream; base and h must be
+ * ready there. For callers whose heights live on the device (a torch optimiser): no host copy of the heights. */
+int arh_geom_set_heights_device(arh_geom *g, const double *base, const int *vids, int HH, const double *h);
 void arh_geom_destroy(arh_geom *g);
 
 /* ------------------------------------------------------------------ H-matrix */
@@ -257,6 +265,12 @@ int arh_diffusion(arh_ctx *ctx, int LL, int nrhs, const double *y, double *d, do
 int arh_shape_grad(arh_geom *g, double k, const double *L, int LL, const int *diff, int HH, int nrhs, const double *x,
                    const double *lam, const double *vf, int src_kind, const double *src, double *A, double *B,
                    double *C);
+
+/* arh_shape_grad that adds the gradient instead of returning its terms: grad[h] (device, HH) += (A - B) + C,
+ * the order of the reference sum A - B + C (adjoint.cpp:232). */
+int arh_shape_grad_accumulate(arh_geom *g, double k, const double *L, int LL, const int *diff, int HH, int nrhs,
+                              const double *x, const double *lam, const double *vf, int src_kind, const double *src,
+                              double *grad);
 
 /* ------------------------------------------------------------------ one band, batched */
 /* The unit of the relief workload for one frequency: assemble, solve nrhs plane waves or point
