@@ -104,3 +104,54 @@ def test_relief_setup_uvs():
     assert uvs.shape == (len(diff), 2) and uvs.min() >= 0.0001 - 1e-15 and uvs.max() <= 0.9999 + 1e-15
     assert np.allclose(uvs, R.rect_points_to_uv(P[diff], *P[:, 0].min(keepdims=True), *P[:, 0].max(keepdims=True),
                                                 *P[:, 2].min(keepdims=True), *P[:, 2].max(keepdims=True)))
+
+
+def test_base_mesh_numbering_by_hand():
+    """A 1 x 1 x 1 segment box enumerated by hand from the reference's loops (pyoptim/mesher.py:113-151):
+    six separate 2 x 2 vertex grids (x fastest along u, then v) and two triangles per face in the order
+    (a, c, d), (a, d, b) for an outward sign, (a, d, c), (a, b, d) otherwise."""
+    w, h, b = 2.0, 0.5, 3.0
+    P, E = mesher.generate_base_mesh(w, h, b, 1, 1, 1)
+    x0, x1, z0, z1 = -1.0, 1.0, -1.5, 1.5
+    Pw = [(x0, 0, z0), (x1, 0, z0), (x0, h, z0), (x1, h, z0),      # start (-w/2, 0, -b/2), u = x, v = y
+          (x0, 0, z1), (x1, 0, z1), (x0, h, z1), (x1, h, z1),      # start (-w/2, 0, b/2)
+          (x0, h, z0), (x1, h, z0), (x0, h, z1), (x1, h, z1),      # start (-w/2, h, -b/2), u = x, v = z
+          (x0, 0, z0), (x1, 0, z0), (x0, 0, z1), (x1, 0, z1),      # start (-w/2, 0, -b/2), u = x, v = z
+          (x0, 0, z0), (x0, h, z0), (x0, 0, z1), (x0, h, z1),      # start (-w/2, 0, -b/2), u = y, v = z
+          (x1, 0, z0), (x1, h, z0), (x1, 0, z1), (x1, h, z1)]      # start (w/2, 0, -b/2), u = y, v = z
+    Ew = [(0, 2, 3), (0, 3, 1), (4, 7, 6), (4, 5, 7), (8, 10, 11), (8, 11, 9),
+          (12, 15, 14), (12, 13, 15), (16, 18, 19), (16, 19, 17), (20, 23, 22), (20, 21, 23)]
+    assert np.array_equal(P, np.array(Pw, dtype=float))
+    assert np.array_equal(E, np.array(Ew))
+
+
+def test_face_subdivision_numbering_by_hand():
+    """face_subdivision on a two-triangle square, enumerated by hand from pyoptim/mesher.py:163-227: the
+    midpoints of the selected face's edges (i1, i2), (i2, i3), (i1, i3) are appended in that order, the
+    face becomes (i1, i4, i6), (i4, i2, i5), (i6, i5, i3), (i4, i5, i6), and the neighbour with one
+    split edge takes the pattern of that edge's position."""
+    P = np.array([(0, 0, 0), (1, 0, 0), (0, 0, 1), (1, 0, 1)], dtype=float)
+    mids = np.array([(0, 0, 0.5), (0.5, 0, 1), (0.5, 0, 0.5)])          # edges (0,2), (2,3), (0,3)
+    four = [[0, 4, 6], [4, 2, 5], [6, 5, 3], [4, 5, 6]]
+    # the shared edge (0, 3) is the neighbour's (i1, i2), (i2, i3) or (i1, i3) edge
+    for nb, want in (((0, 3, 1), [[6, 3, 1], [0, 6, 1]]),
+                     ((1, 0, 3), [[1, 0, 6], [1, 6, 3]]),
+                     ((0, 1, 3), [[0, 1, 6], [1, 3, 6]])):
+        E = np.array([(0, 2, 3), nb])
+        P1, E1, faces = mesher.face_subdivision(P, E, [0], 1, ret_faces=True)
+        assert np.array_equal(P1, np.vstack([P, mids]))
+        assert np.array_equal(E1, np.array(four + want)), (nb, E1)
+        assert list(faces) == [0, 1, 2, 3]
+    # two split edges on a third face: a fan of three triangles around vertex 0, the first two selected;
+    # midpoints (0,1) 4, (1,2) 5, (0,2) 6, (2,3) 7, (0,3) 8
+    P = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)], dtype=float)
+    sel = [[0, 4, 6], [4, 1, 5], [6, 5, 2], [4, 5, 6], [0, 6, 8], [6, 2, 7], [8, 7, 3], [6, 7, 8]]
+    for nb, want in (((0, 3, 1), [[0, 8, 4], [3, 4, 8], [3, 1, 4]]),     # (i2, i3) not split
+                     ((1, 3, 0), [[1, 3, 4], [3, 8, 4], [8, 0, 4]]),     # (i1, i2) not split
+                     ((3, 0, 1), [[3, 8, 1], [1, 8, 4], [8, 0, 4]])):    # (i1, i3) not split
+        E = np.array([(0, 1, 2), (0, 2, 3), nb])
+        P1, E1, faces = mesher.face_subdivision(P, E, [0, 1], 1, ret_faces=True)
+        ends = [(0, 1), (1, 2), (0, 2), (2, 3), (0, 3)]
+        assert np.array_equal(P1, np.vstack([P, [(P[a] + P[b]) / 2 for a, b in ends]]))
+        assert np.array_equal(E1, np.array(sel + want)), (nb, E1)
+        assert list(faces) == list(range(8))
