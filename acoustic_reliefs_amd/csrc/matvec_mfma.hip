@@ -81,14 +81,8 @@ struct Mfma<double> {
 #ifndef ARH_MV_XCD
 #define ARH_MV_XCD 1
 #endif
-#ifndef ARH_MV_LDS
-#define ARH_MV_LDS 0
-#endif
 #ifndef ARH_MV_SLOTS
 #define ARH_MV_SLOTS 2048
-#endif
-#ifndef ARH_MV_LDS_STAGES
-#define ARH_MV_LDS_STAGES 3
 #endif
 // Groups per pipeline stage of a wavefront, for the V-stack (lr) and the row-task kernels (FP64 groups hold
 // twice the registers).
@@ -114,11 +108,6 @@ constexpr int kBigCols = ARH_MV_BIGCOLS;  // stacks with at least this many colu
 constexpr int kPartRows = 64;     // rows of a partial tile for leaves of up to 64 elements
 constexpr int kMaxLeaf = 128;     // the largest leaf the engine takes (8 row tiles; partial tiles of 128 rows)
 constexpr int kTaskSlots = ARH_MV_SLOTS;  // row tasks aimed for: a few per compute unit
-// FP32 row tasks of leaves <= 32 rows stage operands in LDS (global_load_lds). Off: bit for bit the
-// same results, but 126 us against 93 us for the register path on the 4000 Hz / 37 rhs product (one
-// workgroup per CU fits the rings; the scalar source-row loads and the LDS reads are not yet overlapped).
-constexpr bool kLdsRows = ARH_MV_LDS;
-constexpr int kLdsStages = ARH_MV_LDS_STAGES;  // LDS ring depth (stages of groups) per wavefront
 constexpr int kMaxRC = 40;        // right-hand sides per pass (RP <= 40)
 
 // Right-hand side j of a pass is column cols[j0 + j] of x / y (j0 + j without a list).
@@ -406,106 +395,6 @@ __global__ __launch_bounds__(64 * kLrWaves) void k_lr_mfma(const VStack *__restr
         lr_run<T, NT, 1>(S, base, V, xt, red.v, big, w, lane);
 }
 
-// ---------------------------------------------------------------- LDS-staged operands (FP32 rows)
-// Each wavefront keeps a ring of kLdsStages stages in LDS; a stage holds row_depth<float>() groups, a
-// group slot its A columns (4 x m, column-major, room for 16 NRT rows) followed by its 4 rhs rows
-// (4 x RP). Global -> LDS copies (global_load_lds_dwordx4: lane l writes 16 bytes at slot + 16 l)
-// take no registers, so the loads of the next kLdsStages - 1 stages stay in flight while a stage is
-// multiplied; the only wait is an explicit vmcnt for the stage about to be read. Every copy is
-// issued unconditionally for a clamped group index, which keeps the count per stage fixed.
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-// wait until at most n vector-memory operations of this wave are outstanding (gfx9 s_waitcnt fields)
-template <int n>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(n >= 0 && n < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-template <int NT, int NRT>
-struct LdsGeom {
-    static constexpr int RP = 8 * NT;
-    static constexpr int ABYTES = 4 * 16 * NRT * 8;           // A columns, float2
-    static constexpr int BBYTES = 4 * RP * 8;                 // rhs rows
-    static constexpr int SLOT = ABYTES + BBYTES;              // bytes of one group
-    static constexpr int NI = (SLOT + 1023) / 1024;           // copies of 64 x 16 bytes per group
-    static constexpr int STAGE = SLOT * row_depth<float>();   // groups per stage as in the register path
-    static constexpr int WAVE = STAGE * kLdsStages;           // one wave's ring
-};
-
-// Issue the copies of one group into slot (LDS byte address): A = 4 m float2 from acol, rows r[k] of xt.
-template <int NT, int NRT>
-__device__ __forceinline__ void lds_issue(char *slot, const float2 *acol, int m, const int4 &r,
-                                          const float2 *__restrict__ xt, int lane) {
-    typedef LdsGeom<NT, NRT> G;
-    const int abytes = 32 * m;
-#pragma unroll
-    for (int i = 0; i < G::NI; i++) {
-        const int o = i * 1024 + 16 * lane;
-        const char *src = nullptr;
-        if (o < abytes) {
-            src = reinterpret_cast<const char *>(acol) + o;
-        } else if (o >= G::ABYTES && o < G::SLOT) {
-            const int j = o - G::ABYTES, k = j / (G::RP * 8), w = j - k * (G::RP * 8);
-            const int row = k == 0 ? r.x : k == 1 ? r.y : k == 2 ? r.z : r.w;
-            src = reinterpret_cast<const char *>(xt + (int64_t)row * G::RP) + w;
-        }
-        if (src) __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(slot + i * 1024), 16, 0, 0);
-    }
-}
-
-// One group's MFMAs from its LDS slot (same products and order as grp_mma).
-template <int NT, int NRT>
-__device__ __forceinline__ void lds_mma(Acc<float, NT, NRT> &acc, const char *slot, int m, int lane) {
-    typedef LdsGeom<NT, NRT> G;
-    const int i16 = lane & 15, kk = lane >> 4, s = i16 & 1;
-    const float2 *A = reinterpret_cast<const float2 *>(slot) + kk * m;
-    const float *B = reinterpret_cast<const float *>(slot + G::ABYTES) + kk * 2 * G::RP + i16;
-    Grp<float, NT, NRT> g;
-    g.okf = 1.0f;
-#pragma unroll
-    for (int rt = 0; rt < NRT; rt++) g.a[rt] = A[min(rt * 16 + i16, m - 1)];
-#pragma unroll
-    for (int nt = 0; nt < NT; nt++) g.xb[nt] = B[nt * 16];
-    grp_mma<float, NT, NRT, false>(acc, g, s);
-}
-
-// The groups q0, q0 + qstride, ... < qend of a row task from the LDS ring. Groups are taken two per
-// stage in the order ring_run takes them, so the products are the same in the same order (the
-// results are bit for bit those of the register path).
-template <int NT, int NRT>
-__device__ __forceinline__ void lds_ring_run(Acc<float, NT, NRT> &acc, int q0, int qstride, int qend, char *ring,
-                                             const float2 *A0, int m, const int4 *src4,
-                                             const float2 *__restrict__ xt, int lane) {
-    typedef LdsGeom<NT, NRT> G;
-    constexpr int D = row_depth<float>(), S = kLdsStages;
-    if (q0 >= qend) return;
-    const int qlast = qend - 1;
-    const int nst = ((qend - q0 + qstride - 1) / qstride + D - 1) / D;  // stages of this wave
-    auto issue = [&](int st) {
-        char *base = ring + (st % S) * G::STAGE;
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            const int q = min(q0 + (st * D + j) * qstride, qlast);
-            const int4 r = src4[q];
-            lds_issue<NT, NRT>(base + j * G::SLOT, A0 + (int64_t)q * 4 * m, m, r, xt, lane);
-        }
-    };
-#pragma unroll
-    for (int st = 0; st < S - 1; st++) issue(st);
-    for (int st = 0; st < nst; st++) {
-        issue(st + S - 1);
-        wait_vmcnt<(S - 1) * D * G::NI>();
-        __builtin_amdgcn_sched_barrier(0);
-        const char *base = ring + (st % S) * G::STAGE;
-#pragma unroll
-        for (int j = 0; j < D; j++)
-            if (q0 + (st * D + j) * qstride < qend) lds_mma<NT, NRT>(acc, base + j * G::SLOT, m, lane);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    wait_vmcnt<0>();  // the ring's last copies land before the LDS is reused
-}
-
 // ---------------------------------------------------------------- phase 2: row-leaf tasks
 // A task is a range of one row leaf's panel groups, taken by one workgroup: each wave takes a share
 // of the groups (wave_share) and the partial tiles are summed in a fixed order in LDS. A leaf with
@@ -526,11 +415,6 @@ __device__ __forceinline__ void rows_run(const RowTask &T_, int rs, int m, const
     const T2 *A0 = panel + L.off + (int64_t)kk * m;
     const int4 *src4 = reinterpret_cast<const int4 *>(psrc + L.col0) + T_.g0;
     const int3 sh = wave_share<kRowWaves>(T_.g1 - T_.g0, w);
-    if constexpr (kLdsRows && sizeof(T) == 4 && NRT <= 2) {
-        char *ring = reinterpret_cast<char *>(red) + w * LdsGeom<NT, NRT>::WAVE;
-        lds_ring_run<NT, NRT>(acc, sh.x, sh.y, sh.z, ring, panel + L.off + (int64_t)T_.g0 * 4 * m, m, src4, xt, lane);
-        __syncthreads();  // every wave is done with its ring before the reduction reuses the LDS
-    } else
     ring_run<T, NT, NRT, false, row_depth<T>(), int4>(
         acc, sh.x, sh.y, sh.z, s, [&](int q) { return src4[q]; },  // the group's 4 source rows (scalar load)
         [&](int q, const int4 &sr, Grp<T, NT, NRT> &g) {
@@ -560,13 +444,8 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_rows_mfma(
     const int *__restrict__ psrc, const typename Mfma<T>::T2 *__restrict__ panel,
     const typename Mfma<T>::T2 *__restrict__ xt, const double *__restrict__ x, int N, int R, double *__restrict__ y,
     const int *__restrict__ cols, int j0, const int *__restrict__ gate, T *__restrict__ part, int prow) {
-    // the reduction buffer, or (LDS-staged FP32 rows) the wavefronts' rings, whichever is larger
     constexpr int PR = MAXRT > 4 ? kMaxLeaf : kPartRows;  // output staging rows
-    constexpr bool lds_rows = kLdsRows && sizeof(T) == 4 && MAXRT == 2;
-    constexpr size_t ring_bytes = lds_rows ? (size_t)kRowWaves * LdsGeom<NT, 2>::WAVE : 0;
-    constexpr size_t bytes = std::max(sizeof(Lds<T, NT, kRowWaves, PR>), ring_bytes);
-    __shared__ __attribute__((aligned(16))) char smem[bytes];
-    Lds<T, NT, kRowWaves, PR> &red = *reinterpret_cast<Lds<T, NT, kRowWaves, PR> *>(smem);
+    __shared__ Lds<T, NT, kRowWaves, PR> red;
     if (gated(gate)) return;
     const int w = wave_id(), lane = threadIdx.x & 63;
     const RowTask T_ = tasks[blockIdx.x];
