@@ -223,7 +223,7 @@ struct MfmaLayout {
     arh::DBuf panel[2], vs[2], xt[2], part[2];
     bool built[2] = {false, false};
     bool lists = false;
-    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit, d_maps, d_csrc;
+    arh::DBuf d_stacks, d_p1units, d_lpanel, d_psrc, d_rtasks, d_rsplit, d_maps, d_csrc, d_trow;
     std::vector<VStack> stacks;
     std::vector<int2> p1units;
     int p1big = 0, nmaps = 0;
@@ -284,6 +284,10 @@ struct arh_geom_s {
     int64_t aca_u_total = 0, aca_v_total = 0, aca_f_total = 0;
     std::vector<int64_t> us_off, vs_off, flag_off;
     std::vector<int> lim;
+    // ACA launch order (largest work first, workgroup blocks leading): fixed by the partition and
+    // max_rank, so it is computed once per max_rank
+    int aca_order_rank = -1, aca_nbig = 0;
+    arh::DBuf d_aca_order;
 };
 
 struct arh_hmat_s {

@@ -321,7 +321,6 @@ void assemble_aca(arh_hmat *h) {
     if (na == 0) return;
     ARH_CHECK(h->max_rank <= kMaxRankLds, ARH_EINVAL, "max_rank above the supported 128");
     ensure_aca_scratch(g, h->max_rank);
-    std::vector<int> order(na);
     for (int a = 0; a < na; a++) {
         LRBlock &B = h->hblocks[a];
         B.rs = g->lowrank[4 * a], B.re = g->lowrank[4 * a + 1], B.cs = g->lowrank[4 * a + 2];
@@ -329,32 +328,38 @@ void assemble_aca(arh_hmat *h) {
         B.lim = g->lim[a];
         B.l0 = g->leaf_of[B.rs];
         B.us_off = g->us_off[a], B.vs_off = g->vs_off[a], B.flag_off = g->flag_off[a];
-        order[a] = a;
     }
-    // largest work first: the ACA cost grows like lim^2 (m + n)
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const LRBlock &X = h->hblocks[x], &Y = h->hblocks[y];
-        double wx = (double)X.lim * X.lim * (X.re - X.rs + X.ce - X.cs) + (X.re - X.rs + X.ce - X.cs);
-        double wy = (double)Y.lim * Y.lim * (Y.re - Y.rs + Y.ce - Y.cs) + (Y.re - Y.rs + Y.ce - Y.cs);
-        return wx > wy;
-    });
-    // blocks with m + n >= kAcaBigDim get a workgroup each (they lead the order, largest first);
-    // the rest one wavefront each
-    std::stable_partition(order.begin(), order.end(), [&](int x) {
-        const LRBlock &X = h->hblocks[x];
-        return X.re - X.rs + X.ce - X.cs >= kAcaBigDim;
-    });
-    int nbig = 0;
-    while (nbig < na && h->hblocks[order[nbig]].re - h->hblocks[order[nbig]].rs + h->hblocks[order[nbig]].ce -
-                                h->hblocks[order[nbig]].cs >=
-                            kAcaBigDim)
-        nbig++;
+    std::vector<int> order;  // kept until the synchronisation below (the upload may read it late)
+    if (g->aca_order_rank != h->max_rank) {
+        order.resize(na);
+        for (int a = 0; a < na; a++) order[a] = a;
+        // largest work first: the ACA cost grows like lim^2 (m + n)
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const LRBlock &X = h->hblocks[x], &Y = h->hblocks[y];
+            double wx = (double)X.lim * X.lim * (X.re - X.rs + X.ce - X.cs) + (X.re - X.rs + X.ce - X.cs);
+            double wy = (double)Y.lim * Y.lim * (Y.re - Y.rs + Y.ce - Y.cs) + (Y.re - Y.rs + Y.ce - Y.cs);
+            return wx > wy;
+        });
+        // blocks with m + n >= kAcaBigDim get a workgroup each (they lead the order, largest first);
+        // the rest one wavefront each
+        std::stable_partition(order.begin(), order.end(), [&](int x) {
+            const LRBlock &X = h->hblocks[x];
+            return X.re - X.rs + X.ce - X.cs >= kAcaBigDim;
+        });
+        int nbig = 0;
+        while (nbig < na && h->hblocks[order[nbig]].re - h->hblocks[order[nbig]].rs + h->hblocks[order[nbig]].ce -
+                                    h->hblocks[order[nbig]].cs >=
+                                kAcaBigDim)
+            nbig++;
+        upload(g->d_aca_order, order, s);
+        g->aca_nbig = nbig;
+        g->aca_order_rank = h->max_rank;
+    }
+    const int nbig = g->aca_nbig;
     upload(h->blocks, h->hblocks, s);
-    DBuf d_order;
-    upload(d_order, order, s);
     const int grid = nbig + (na - nbig + kAcaWaves - 1) / kAcaWaves;
-    hipLaunchKernelGGL(k_aca, dim3(grid), dim3(kAcaThreads), 0, s, h->blocks.as<LRBlock>(), d_order.as<int>(), nbig,
-                       na, g->tri.as<Tri>(), h->k, h->aca_tol, g->aca_U.as<double2>(), g->aca_V.as<double2>(),
+    hipLaunchKernelGGL(k_aca, dim3(grid), dim3(kAcaThreads), 0, s, h->blocks.as<LRBlock>(), g->d_aca_order.as<int>(),
+                       nbig, na, g->tri.as<Tri>(), h->k, h->aca_tol, g->aca_U.as<double2>(), g->aca_V.as<double2>(),
                        g->aca_flags.as<unsigned char>());
     ARH_HIP(hipGetLastError());
     ARH_HIP(hipMemcpyAsync(h->hblocks.data(), h->blocks.p, sizeof(LRBlock) * na, hipMemcpyDeviceToHost, s));

@@ -26,7 +26,7 @@
 // Lane layouts (64 lanes, 16x16x4): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; D[i][j] in
 // lane j + 16 (i / 4), element i % 4 for f32, lane j + 16 (i % 4), element i / 4 for f64.
 #include <algorithm>
-#include <map>
+#include <unordered_map>
 
 #include "arh_internal.h"
 #include "device_common.h"
@@ -573,6 +573,98 @@ __global__ __launch_bounds__(256) void k_fuse_panel(const int *__restrict__ leaf
     }
 }
 
+// Panel column lists of one leaf, written by one wavefront: the source row (psrc: x row, or N + the
+// intermediates row) and the FP64 gather source (csrc) of every column of the leaf's fused panel.
+//  forward:    the row leaf's dense columns (panel column c of the dense panel), then for each low-rank
+//              block covering the leaf (lr list order) its rank U columns restricted to the leaf;
+//  transposed: for each dense block in the column leaf's list, its mr rows as columns (a row of the
+//              column-major block in its row leaf's panel, stride mr), then the V columns restricted to
+//              the leaf of each low-rank block whose column range covers it;
+// zero columns (source row 0, kind 0) up to the leaf's ncol. A list's column offsets are a wavefront
+// prefix sum of the entries' widths. bad[0] is raised when a column count disagrees with the host's
+// ncol, a source row falls outside [0, nsrc), or a low-rank block does not cover the leaf.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < ARH_WAVE; d <<= 1) {
+        const int t = __shfl_up(v, d, ARH_WAVE);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(ARH_WAVE) void k_panel_cols(
+    int tr, const LeafPanel *__restrict__ lpanel, const int *__restrict__ leaf_start,
+    const int *__restrict__ pcol_ptr, const int64_t *__restrict__ panel_off, const int *__restrict__ pcol_idx,
+    const int *__restrict__ dT_ptr, const int *__restrict__ dT_blk, const int *__restrict__ dblk_info,
+    const int *__restrict__ lr_ptr, const int *__restrict__ lr_blk,
+    const LRBlock *__restrict__ blocks, const int *__restrict__ t_row, int nsrc, int *__restrict__ psrc,
+    ColSrc *__restrict__ csrc, int *__restrict__ bad) {
+    const int l = blockIdx.x, lane = threadIdx.x;
+    const int ls = leaf_start[l], m = leaf_start[l + 1] - ls;
+    const LeafPanel L = lpanel[l];
+    const int ncol = L.ngroups * 4;
+    int *ps = psrc + L.col0;
+    ColSrc *cs = csrc + L.col0;
+    int base = 0;
+    bool ok = true;
+    if (!tr) {
+        const int p0 = pcol_ptr[l], nd = pcol_ptr[l + 1] - p0;
+        for (int c = lane; c < nd && c < ncol; c += ARH_WAVE) {
+            ps[c] = pcol_idx[p0 + c];
+            cs[c] = ColSrc{panel_off[l] + (int64_t)c * m, 1, 1};
+        }
+        base = nd;
+    } else {
+        const int e0 = dT_ptr[l], e1 = dT_ptr[l + 1];
+        for (int c0 = e0; c0 < e1; c0 += ARH_WAVE) {
+            const int e = c0 + lane;
+            int b = 0, rl = 0, pc = 0, mr = 0;
+            if (e < e1) {
+                b = dT_blk[e];
+                rl = dblk_info[2 * b], pc = dblk_info[2 * b + 1];
+                mr = leaf_start[rl + 1] - leaf_start[rl];
+            }
+            const int inc = wave_incl_scan(mr, lane);
+            const int start = base + inc - mr;
+            const int rs = e < e1 ? leaf_start[rl] : 0;
+            for (int j = 0; j < mr && start + j < ncol; j++) {
+                ps[start + j] = rs + j;
+                cs[start + j] = ColSrc{panel_off[rl] + (int64_t)pc * mr + j, mr, 1};
+            }
+            base += __shfl(inc, ARH_WAVE - 1, ARH_WAVE);
+        }
+    }
+    const int e0 = lr_ptr[l], e1 = lr_ptr[l + 1];
+    for (int c0 = e0; c0 < e1; c0 += ARH_WAVE) {
+        const int e = c0 + lane;
+        int r = 0, b = 0;
+        LRBlock B{};
+        if (e < e1) {
+            b = lr_blk[e];
+            B = blocks[b];
+            r = B.rank;
+            if (tr ? (ls < B.cs || ls + m > B.ce) : (ls < B.rs || ls + m > B.re)) ok = false;
+        }
+        const int inc = wave_incl_scan(r, lane);
+        const int start = base + inc - r;
+        for (int p = 0; p < r && start + p < ncol; p++) {
+            const int src = t_row[b] + p;
+            ps[start + p] = src;
+            if (src < 0 || src >= nsrc) ok = false;
+            cs[start + p] = tr ? ColSrc{B.v_off + (int64_t)p * (B.ce - B.cs) + (ls - B.cs), 1, 3}
+                               : ColSrc{B.u_off + (int64_t)(ls - B.rs) * B.rank + (int64_t)p * m, 1, 2};
+        }
+        base += __shfl(inc, ARH_WAVE - 1, ARH_WAVE);
+    }
+    // the host rounded the leaf's column count up to a multiple of 4
+    if (base > ncol || base + 4 <= ncol) ok = false;
+    for (int c = base + lane; c < ncol; c += ARH_WAVE) {
+        ps[c] = 0;
+        cs[c] = ColSrc{0, 0, 0};
+    }
+    if (!ok) atomicOr(bad, 1);
+}
+
 // Work lists of one orientation, shared by both precisions (host + device copies).
 //  forward:    stacks of V by column cluster; panels of row leaves: dense columns, then the U
 //              columns (leaf segment) of the low-rank blocks covering the leaf.
@@ -586,27 +678,43 @@ static void build_lists(arh_hmat *h, int tr) {
     const int N = g->N;
     ARH_CHECK(g->max_leaf <= kMaxLeaf, ARH_EINVAL, "MFMA product supports leaves of at most 128 elements");
     // stacks: active low-rank blocks grouped by their column (forward) / row (transposed) cluster,
-    // in order of first appearance
-    auto cluster = [&](const LRBlock &B) { return tr ? std::make_pair(B.rs, B.re) : std::make_pair(B.cs, B.ce); };
-    std::map<std::pair<int, int>, int> sid;
-    std::vector<std::vector<int>> members;
-    for (int a : h->active_lr) {
-        auto it = sid.emplace(cluster(h->hblocks[a]), (int)members.size());
-        if (it.second) members.emplace_back();
-        members[it.first->second].push_back(a);
+    // stacks in order of first appearance, members in active-block order
+    auto key = [&](const LRBlock &B) {
+        return tr ? ((int64_t)B.rs << 32) | (uint32_t)B.re : ((int64_t)B.cs << 32) | (uint32_t)B.ce;
+    };
+    const int nact = (int)h->active_lr.size();
+    std::unordered_map<int64_t, int> sid;
+    sid.reserve(2 * (size_t)nact + 1);
+    std::vector<int> bsid(nact), first;
+    for (int q = 0; q < nact; q++) {
+        auto it = sid.emplace(key(h->hblocks[h->active_lr[q]]), (int)first.size());
+        if (it.second) first.push_back(q);
+        bsid[q] = it.first->second;
+    }
+    const int ns = (int)first.size();
+    std::vector<int> mptr(ns + 1, 0), mem(nact);
+    for (int q = 0; q < nact; q++) mptr[bsid[q] + 1]++;
+    for (int k = 0; k < ns; k++) mptr[k + 1] += mptr[k];
+    {
+        std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+        for (int q = 0; q < nact; q++) mem[fill[bsid[q]]++] = h->active_lr[q];
     }
     M.stacks.clear();
+    M.stacks.reserve(ns);
     std::vector<StackMap> maps;
-    std::vector<int64_t> t_row(h->hblocks.size(), -1);
+    maps.reserve(nact);
+    std::vector<int> t_row(h->hblocks.size(), -1);
     int64_t voff = 0, trow = N;
-    for (auto &mem : members) {
-        const auto cl = cluster(h->hblocks[mem[0]]);
-        VStack S{cl.first, cl.second - cl.first, 0, 0, voff, trow};
-        for (int a : mem) S.rtot += h->hblocks[a].rank;
+    for (int k = 0; k < ns; k++) {
+        const LRBlock &B0 = h->hblocks[mem[mptr[k]]];
+        VStack S{tr ? B0.rs : B0.cs, tr ? B0.re - B0.rs : B0.ce - B0.cs, 0, 0, voff, trow};
+        for (int q = mptr[k]; q < mptr[k + 1]; q++) S.rtot += h->hblocks[mem[q]].rank;
+        ARH_CHECK(trow + S.rtot < INT32_MAX, ARH_EINVAL, "stacked intermediates exceed the 32-bit row range");
         int o = 0;
-        for (int a : mem) {
+        for (int q = mptr[k]; q < mptr[k + 1]; q++) {
+            const int a = mem[q];
             maps.push_back(StackMap{a, S.rtot, voff + o});
-            t_row[a] = trow + o;
+            t_row[a] = (int)(trow + o);
             o += h->hblocks[a].rank;
         }
         M.stacks.push_back(S);
@@ -614,7 +722,7 @@ static void build_lists(arh_hmat *h, int tr) {
         trow += S.rtot;
     }
     M.vs_total = voff;
-    ARH_CHECK(trow - N == h->t_total && trow < INT32_MAX, ARH_EINVAL, "stacked intermediates do not match the ranks");
+    ARH_CHECK(trow - N == h->t_total, ARH_EINVAL, "stacked intermediates do not match the ranks");
     // phase-1 units: big stacks first, each list by decreasing column count
     std::vector<int2> big, small;
     for (int k = 0; k < (int)M.stacks.size(); k++)
@@ -629,59 +737,46 @@ static void build_lists(arh_hmat *h, int tr) {
     M.p1big = (int)big.size();
     M.p1units = big;
     M.p1units.insert(M.p1units.end(), small.begin(), small.end());
-    // fused panels
+    // fused panels: column counts here, the column lists on the device (k_panel_cols)
     M.lpanel.assign(g->nleaf, LeafPanel{0, 0, 0});
-    std::vector<int> psrc;
-    std::vector<ColSrc> csrc;
-    int64_t poff = 0;
+    int64_t poff = 0, ncols = 0;
     for (int l = 0; l < g->nleaf; l++) {
         const int ls = g->leaf_start[l], m = g->leaf_start[l + 1] - ls;
-        const int col0 = (int)psrc.size();
+        int64_t nc = 0;
         if (!tr) {
-            const int nd = g->pcol_ptr[l + 1] - g->pcol_ptr[l];
-            for (int c = 0; c < nd; c++) {
-                psrc.push_back(g->pcol_idx[g->pcol_ptr[l] + c]);
-                csrc.push_back(ColSrc{g->panel_off[l] + (int64_t)c * m, 1, 1});
-            }
-            for (int e = g->lrR_ptr[l]; e < g->lrR_ptr[l + 1]; e++) {
-                const int b = g->lrR_blk[e];
-                const LRBlock &B = h->hblocks[b];
-                ARH_CHECK(ls >= B.rs && ls + m <= B.re, ARH_EINVAL, "low-rank block does not cover its row leaf");
-                for (int p = 0; p < B.rank; p++) {
-                    psrc.push_back((int)(t_row[b] + p));
-                    csrc.push_back(ColSrc{B.u_off + (int64_t)(ls - B.rs) * B.rank + (int64_t)p * m, 1, 2});
-                }
-            }
+            nc = g->pcol_ptr[l + 1] - g->pcol_ptr[l];
+            for (int e = g->lrR_ptr[l]; e < g->lrR_ptr[l + 1]; e++) nc += h->hblocks[g->lrR_blk[e]].rank;
         } else {
             for (int e = g->dT_ptr[l]; e < g->dT_ptr[l + 1]; e++) {
-                const int b = g->dT_blk[e];
-                const int rl = g->dblk_leafr[b], pc = g->dblk_col[b];
-                const int rs = g->leaf_start[rl], mr = g->leaf_start[rl + 1] - rs;
+                const int b = g->dT_blk[e], rl = g->dblk_leafr[b];
                 ARH_CHECK(g->dense[4 * b + 2] == ls && g->dense[4 * b + 3] == ls + m, ARH_EINVAL,
                           "dense block is not a leaf column");
-                for (int j = 0; j < mr; j++) {
-                    psrc.push_back(rs + j);
-                    csrc.push_back(ColSrc{g->panel_off[rl] + (int64_t)pc * mr + j, mr, 1});
-                }
+                nc += g->leaf_start[rl + 1] - g->leaf_start[rl];
             }
-            for (int e = g->lrC_ptr[l]; e < g->lrC_ptr[l + 1]; e++) {
-                const int b = g->lrC_blk[e];
-                const LRBlock &B = h->hblocks[b];
-                ARH_CHECK(ls >= B.cs && ls + m <= B.ce, ARH_EINVAL, "low-rank block does not cover its column leaf");
-                const int n = B.ce - B.cs;
-                for (int p = 0; p < B.rank; p++) {
-                    psrc.push_back((int)(t_row[b] + p));
-                    csrc.push_back(ColSrc{B.v_off + (int64_t)p * n + (ls - B.cs), 1, 3});
-                }
-            }
+            for (int e = g->lrC_ptr[l]; e < g->lrC_ptr[l + 1]; e++) nc += h->hblocks[g->lrC_blk[e]].rank;
         }
-        while ((psrc.size() - col0) % 4) psrc.push_back(0), csrc.push_back(ColSrc{0, 0, 0});
-        const int ncol = (int)psrc.size() - col0;
-        M.lpanel[l] = LeafPanel{poff, col0, ncol / 4};
-        poff += (int64_t)ncol * m;
+        nc = (nc + 3) / 4 * 4;
+        ARH_CHECK(ncols + nc < INT32_MAX, ARH_EINVAL, "panel columns exceed the 32-bit range");
+        M.lpanel[l] = LeafPanel{poff, (int)ncols, (int)(nc / 4)};
+        ncols += nc;
+        poff += nc * m;
     }
     M.panel_total = poff;
-    for (int v : psrc) ARH_CHECK(v >= 0 && v < N + h->t_total, ARH_EINVAL, "panel source row out of range");
+    upload(M.d_lpanel, M.lpanel, s);
+    upload(M.d_trow, t_row, s);
+    M.d_psrc.alloc(sizeof(int) * std::max<int64_t>(1, ncols));
+    M.d_csrc.alloc(sizeof(ColSrc) * std::max<int64_t>(1, ncols));
+    DBuf bad;
+    bad.alloc(sizeof(int));
+    ARH_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_panel_cols, dim3(g->nleaf), dim3(ARH_WAVE), 0, s, tr, M.d_lpanel.as<LeafPanel>(),
+                       g->d_leaf_start.as<int>(), g->d_pcol_ptr.as<int>(), g->d_panel_off.as<int64_t>(),
+                       g->d_pcol_idx.as<int>(), g->d_dT_ptr.as<int>(), g->d_dT_blk.as<int>(),
+                       g->d_dblk_info.as<int>(), tr ? g->d_lrC_ptr.as<int>() : g->d_lrR_ptr.as<int>(),
+                       tr ? g->d_lrC_blk.as<int>() : g->d_lrR_blk.as<int>(), h->blocks.as<LRBlock>(),
+                       M.d_trow.as<int>(), (int)(N + h->t_total), M.d_psrc.as<int>(), M.d_csrc.as<ColSrc>(),
+                       bad.as<int>());
+    ARH_HIP(hipGetLastError());
     // row tasks: leaves with more than tg groups are split evenly
     int64_t gt = 0;
     for (auto &L : M.lpanel) gt += L.ngroups;
@@ -722,14 +817,14 @@ static void build_lists(arh_hmat *h, int tr) {
     }
     upload(M.d_stacks, M.stacks, s);
     upload(M.d_p1units, M.p1units, s);
-    upload(M.d_lpanel, M.lpanel, s);
-    upload(M.d_psrc, psrc, s);
     upload(M.d_rtasks, M.rtasks, s);
     upload(M.d_rsplit, M.rsplit, s);
     upload(M.d_maps, maps, s);
-    upload(M.d_csrc, csrc, s);
     M.nmaps = (int)maps.size();
+    int nbad = 0;
+    ARH_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     ARH_HIP(hipStreamSynchronize(s));  // the host vectors above are freed on return
+    ARH_CHECK(!nbad, ARH_EINVAL, "panel column lists disagree with the partition or the ranks");
     M.lists = true;
 }
 

@@ -67,8 +67,9 @@ def run_mixed(g, bands, dirs, L, capi, lat):
                                          tol=1e-6, true_residual=True)
         dt = time.perf_counter() - t0
         lat.extend([dt] * len(dirs))
-        log("  mixed band %g Hz: %.3f s, iters %d, refinements %d, true residual %.2e, max rank %d" % (
-            f, dt, st["iters"], st["refinements"], st["max_true_res"], info["max_rank"]))
+        log("  mixed band %g Hz: %.3f s (assembly %.1f ms device, solve %.1f ms), iters %d, refinements %d, "
+            "true residual %.2e, max rank %d" % (f, dt, info["assemble_ms"], st["solve_ms"], st["iters"],
+                                                 st["refinements"], st["max_true_res"], info["max_rank"]))
         ds.append((d, st, info))
     return ds
 
@@ -86,9 +87,10 @@ def run_served(srv, mid, bands, dirs, L, capi, lat):
         rid = rids[f]
         d, info = srv.wait(rid)
         lat.extend([info["t_done"] - info["t_submit"]] * len(dirs))
-        log("  served band %g Hz: queued %.3f s, service %.3f s, iters %d, true residual %.2e" % (
-            f, info["t_start"] - info["t_submit"], info["t_done"] - info["t_start"], info["iters"],
-            info["max_true_res"]))
+        log("  served band %g Hz: queued %.3f s, service %.3f s (assembly %.1f ms device, solve %.1f ms, receivers "
+            "%.1f ms), iters %d, true residual %.2e" % (
+                f, info["t_start"] - info["t_submit"], info["t_done"] - info["t_start"], info["assemble_ms"],
+                info["solve_ms"], info["post_ms"], info["iters"], info["max_true_res"]))
         out.append((d, info, None))
     return out
 
