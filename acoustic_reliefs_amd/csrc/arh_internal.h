@@ -2,12 +2,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/arh.h"
+#include "trig.h"
 
 namespace arh {
 
@@ -28,6 +30,12 @@ struct Error : std::runtime_error {
     do {                                                        \
         if (!(cond)) throw ::arh::Error((code), (msg));          \
     } while (0)
+
+// A wavenumber the kernels accept: finite, >= 0 and at most kTrigMaxK, so that every phase k r they
+// form stays inside the domain of their sine and cosine (trig.h).
+#define ARH_CHECK_K(k)                                                                               \
+    ARH_CHECK(std::isfinite(k) && (k) >= 0 && (k) <= ::arh::kTrigMaxK, ARH_EINVAL,                    \
+              "wavenumber must be finite, >= 0 and <= 1e6")
 
 // Device memory cache (pool.cpp). dev_alloc rounds n up to the block it hands out; the block is
 // ordered after its previous owner's last use when the streams differ. tl_stream is the stream of

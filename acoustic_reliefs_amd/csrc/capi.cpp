@@ -331,7 +331,7 @@ static void fill_info(arh_hmat *h) {
 int arh_hmat_assemble(arh_geom *g, double k, double aca_tol, int max_rank, int storage, arh_hmat **out) {
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && out, ARH_EINVAL, "NULL argument");
-        ARH_CHECK(std::isfinite(k) && k >= 0, ARH_EINVAL, "wavenumber must be finite and >= 0");
+        ARH_CHECK_K(k);
         ARH_CHECK(max_rank >= 0, ARH_EINVAL, "max_rank must be >= 0");
         ARH_CHECK(storage == ARH_FP64 || storage == ARH_MIXED, ARH_EINVAL, "storage must be ARH_FP64 or ARH_MIXED");
         set_device(g->ctx);
@@ -470,6 +470,7 @@ int arh_rhs(arh_geom *g, double k, int kind, int nrhs, const double *vecs, doubl
         ARH_CHECK(g && vecs && b, ARH_EINVAL, "NULL argument");
         ARH_CHECK(kind == ARH_RHS_POINT || kind == ARH_RHS_PLANE, ARH_EINVAL, "unknown rhs kind");
         ARH_CHECK(nrhs >= 1, ARH_EINVAL, "nrhs must be >= 1");
+        ARH_CHECK_K(k);
         set_device(g->ctx);
         DBuf v;
         std::vector<double> hv(vecs, vecs + 3 * (size_t)nrhs);
@@ -484,6 +485,7 @@ int arh_receivers(arh_geom *g, double k, const double *L, int LL, int transpose,
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && L && x && y, ARH_EINVAL, "NULL argument");
         ARH_CHECK(LL >= 1 && nrhs >= 1, ARH_EINVAL, "LL and nrhs must be >= 1");
+        ARH_CHECK_K(k);
         set_device(g->ctx);
         DBuf Ld;
         std::vector<double> hL(L, L + 3 * (size_t)LL);
@@ -497,6 +499,7 @@ int arh_receivers_noalloc(arh_geom *g, double k, const double *L, int LL, int tr
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && L && x && y, ARH_EINVAL, "NULL argument");
         ARH_CHECK(LL >= 1 && nrhs >= 1, ARH_EINVAL, "LL and nrhs must be >= 1");
+        ARH_CHECK_K(k);
         set_device(g->ctx);
         launch_receivers_noalloc(g, k, L, LL, transpose, nrhs, (const double2 *)x, (double2 *)y);
     });
@@ -507,7 +510,8 @@ int arh_qaca_create(arh_geom *g, double k, const double *L, int LL, double tol, 
         ARH_CHECK(g && L && out, ARH_EINVAL, "NULL argument");
         ARH_CHECK(LL >= 1, ARH_EINVAL, "LL must be >= 1");
         ARH_CHECK(max_rank >= 0, ARH_EINVAL, "max_rank must be >= 0");
-        ARH_CHECK(std::isfinite(k) && k > 0, ARH_EINVAL, "k must be finite and > 0");
+        ARH_CHECK_K(k);
+        ARH_CHECK(k > 0, ARH_EINVAL, "k must be > 0");
         set_device(g->ctx);
         auto q = std::make_unique<arh_qaca>();
         q->g = g, q->k = k, q->tol = tol, q->LL = LL, q->N = g->N;
@@ -618,6 +622,7 @@ int arh_shape_grad(arh_geom *g, double k, const double *L, int LL, const int *di
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && L && diff && x && lam && vf && src && A && B && C, ARH_EINVAL, "NULL argument");
         ARH_CHECK(HH >= 0 && nrhs >= 1 && LL >= 1, ARH_EINVAL, "bad sizes");
+        ARH_CHECK_K(k);
         set_device(g->ctx);
         DBuf Ld, Sd;
         upload(Ld, std::vector<double>(L, L + 3 * (size_t)LL), g->ctx->stream);
@@ -633,6 +638,7 @@ int arh_shape_grad_accumulate(arh_geom *g, double k, const double *L, int LL, co
     return guard(g ? g->ctx : nullptr, [&] {
         ARH_CHECK(g && L && diff && x && lam && vf && src && grad, ARH_EINVAL, "NULL argument");
         ARH_CHECK(HH >= 0 && nrhs >= 1 && LL >= 1, ARH_EINVAL, "bad sizes");
+        ARH_CHECK_K(k);
         set_device(g->ctx);
         DBuf Ld, Sd;
         upload(Ld, std::vector<double>(L, L + 3 * (size_t)LL), g->ctx->stream);

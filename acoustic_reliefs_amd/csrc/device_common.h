@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "trig.h"
+
 #define ARH_WAVE 64
 
 namespace arh {
@@ -55,7 +57,7 @@ __device__ __forceinline__ double2 bem_entry(double k, double px, double py, dou
         double s = -(dx * t.nx + dy * t.ny + dz * t.nz);
         double kr = k * r;
         double sn, cs;
-        sincos(kr, &sn, &cs);
+        trig_sincos(kr, &sn, &cs);
         double f = w * s / (r2 * r);
         re += f * (-cs - kr * sn);
         im += f * (sn - kr * cs);

@@ -34,7 +34,7 @@ constexpr int kRQ = ARH_GRAD_RQ;
 
 __device__ __forceinline__ void FdF(double k, double r, double2 &F, double2 &dF) {
     double sn, cs;
-    sincos(k * r, &sn, &cs);
+    trig_sincos(k * r, &sn, &cs);
     const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2;
     const double kr = k * r;
     const double inv4pi = 1.0 / (4.0 * kPi);
@@ -250,13 +250,13 @@ __global__ __launch_bounds__(kGT) void k_grad_AC(const Tri *__restrict__ tri, in
                 const double dx = ti.gx - sx, dy = ti.gy - sy, dz = ti.gz - sz;
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 double sn, cs;
-                sincos(k * r, &sn, &cs);
+                trig_sincos(k * r, &sn, &cs);
                 // e^{-ikr}(-1 - ikr) / (4 pi r^2) * d_y / (3 r)
                 const double f = dy / (3.0 * r) / (4.0 * kPi * r * r);
                 db = make_double2((-cs - k * r * sn) * f, (sn - k * r * cs) * f);
             } else {
                 double sn, cs;
-                sincos(k * (sx * ti.gx + sy * ti.gy + sz * ti.gz), &sn, &cs);
+                trig_sincos(k * (sx * ti.gx + sy * ti.gy + sz * ti.gz), &sn, &cs);
                 // (i f) e^{-i k d.c} with f = -k d_y / 3
                 const double f = -k * sy / 3.0;
                 db = make_double2(f * sn, f * cs);

@@ -22,13 +22,13 @@ __global__ void k_rhs(const Tri *__restrict__ tri, int N, double k, int kind, co
         double dx = t.gx - vx, dy = t.gy - vy, dz = t.gz - vz;
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double sn, cs;
-        sincos(k * r, &sn, &cs);
+        trig_sincos(k * r, &sn, &cs);
         double f = 1.0 / (4.0 * kPi * r);
         out = make_double2(cs * f, -sn * f);
     } else {
         double ph = k * (vx * t.gx + vy * t.gy + vz * t.gz);
         double sn, cs;
-        sincos(ph, &sn, &cs);
+        trig_sincos(ph, &sn, &cs);
         out = make_double2(cs, -sn);
     }
     b[(int64_t)j * N + i] = out;

@@ -278,6 +278,7 @@ int arh_server_submit(arh_server *s, int mesh_id, double k, double aca_tol, int 
         ARH_CHECK(nrhs >= 1 && LL >= 2, ARH_EINVAL, "nrhs >= 1 and LL >= 2 required");
         ARH_CHECK(kind == ARH_RHS_POINT || kind == ARH_RHS_PLANE, ARH_EINVAL, "unknown rhs kind");
         ARH_CHECK(storage == ARH_FP64 || storage == ARH_MIXED, ARH_EINVAL, "unknown storage");
+        ARH_CHECK_K(k);
         std::unique_lock<std::mutex> lk(s->mu);
         ARH_CHECK(mesh_id >= 0 && mesh_id < (int)s->meshes.size(), ARH_EINVAL, "unknown mesh id");
         for (auto &e : s->worker_err) ARH_CHECK(e.empty(), ARH_EHIP, "server worker failed: " + e);

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_finish(RhsState *__restrict__ S, int R,
                                                 const double *__restrict__ part_ts, const double *__restrict__ part_rr,
                                                 double *__restrict__ hist, int hist_cap, int *__restrict__ gate,
                                                 int *__restrict__ gate_hist) {
-    __shared__ int cnt[256];
+    __shared__ int cnt[256 / ARH_WAVE];
     const double eps2 = 4.930380657631324e-32;  // (2^-52)^2, Eigen's NumTraits<double>::epsilon() squared
     int nact = 0;
     for (int j = threadIdx.x; j < R; j += blockDim.x) {
@@ -235,11 +235,13 @@ __global__ __launch_bounds__(256) void k_finish(RhsState *__restrict__ S, int R,
         }
         S[j] = st;
     }
-    cnt[threadIdx.x] = nact;
+    // integer count: the same whatever the order of the additions
+    for (int off = ARH_WAVE / 2; off > 0; off >>= 1) nact += __shfl_xor(nact, off, ARH_WAVE);
+    if (threadIdx.x % ARH_WAVE == 0) cnt[threadIdx.x / ARH_WAVE] = nact;
     __syncthreads();
     if (threadIdx.x == 0) {
         int c = 0;
-        for (int k = 0; k < (int)blockDim.x; k++) c += cnt[k];
+        for (int k = 0; k < (int)blockDim.x / ARH_WAVE; k++) c += cnt[k];
         gate[0] = c;
         gate_hist[0] = c;
     }
@@ -375,7 +377,11 @@ struct Work {
     }
 };
 
-int pick_nb(int N) { return std::max(1, std::min(kMaxNB, (N + 2047) / 2048)); }
+#ifndef ARH_SOLVER_ROWS
+#define ARH_SOLVER_ROWS 2048
+#endif
+// blocks per right-hand side of the vector kernels: about ARH_SOLVER_ROWS rows each
+int pick_nb(int N) { return std::max(1, std::min(kMaxNB, (N + ARH_SOLVER_ROWS - 1) / ARH_SOLVER_ROWS)); }
 
 }  // namespace
 

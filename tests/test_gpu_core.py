@@ -40,6 +40,18 @@ def test_dense_blocks(sphere_setup):
     assert worst < 1e-12
 
 
+def test_wavenumber_domain(sphere_setup, capi):
+    """The kernels' sine and cosine are accurate for phases up to 1e9: the entry points reject a
+    wavenumber above 1e6 (or negative / not finite) before anything is launched."""
+    g, h, oc = sphere_setup
+    for k in (2e6, -1.0, float("nan")):
+        with pytest.raises(RuntimeError, match="wavenumber"):
+            g.assemble(k, 1e-4, 100, capi.ARH_FP64)
+        with pytest.raises(RuntimeError, match="wavenumber"):
+            g.rhs(k, capi.ARH_RHS_PLANE, [0.0, 0.0, -1.0])
+    g.rhs(1e6, capi.ARH_RHS_PLANE, [0.0, 0.0, -1.0])  # the limit itself is accepted
+
+
 def test_aca_factors(sphere_setup, oracle):
     g, h, oc = sphere_setup
     A = oc.dense_matrix(K500) + 0.5 * np.eye(g.N)
