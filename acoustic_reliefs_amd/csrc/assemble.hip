@@ -11,7 +11,7 @@ constexpr int kAcaThreads = 256;
 constexpr int kAcaWaves = kAcaThreads / ARH_WAVE;
 constexpr int kMaxRankLds = 128;  // max ACA rank kept in LDS coefficient arrays
 #ifndef ARH_ACA_BIGDIM
-#define ARH_ACA_BIGDIM 1024
+#define ARH_ACA_BIGDIM 512
 #endif
 constexpr int kAcaBigDim = ARH_ACA_BIGDIM;  // blocks with m + n at least this get a workgroup, smaller ones a wavefront
 
