@@ -16,6 +16,8 @@
  *    arh_partition (acoustics3d.cpp:47-48).
  *  - A context owns one HIP stream; a context and its objects are not re-entrant, different
  *    contexts are independent.
+ *  - Wavenumbers k (rad/m) must be finite, >= 0 and at most 1e6: the kernels' FP64 sine and cosine
+ *    reduce arguments k r accurately up to 1e9 (r in metres); larger k is rejected with ARH_EINVAL.
  *
  * Reference interfaces replaced (mickey1356/acoustic_reliefs, acoustics3d_cpp/):
  *  see the comment on each entry point.
