@@ -19,6 +19,19 @@ import os
 import sys
 import time
 
+# Hardware queues of this process (HIP's GPU_MAX_HW_QUEUES; 4 when unset). The served run's four worker
+# streams share the process with the main thread's context stream and the runtime's own stream: on 4 queues
+# two of the workers end up on one queue and take turns (two bands finish early, two late). 8 gives every
+# stream a queue of its own. The HIP runtime reads the variable when it starts, so it is set here, before
+# anything touches the GPU; --hw-queues chooses another value.
+HW_QUEUES = "8"
+for _i, _a in enumerate(sys.argv):
+    if _a == "--hw-queues" and _i + 1 < len(sys.argv):
+        HW_QUEUES = sys.argv[_i + 1]
+    elif _a.startswith("--hw-queues="):
+        HW_QUEUES = _a.split("=", 1)[1]
+os.environ["GPU_MAX_HW_QUEUES"] = HW_QUEUES
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -603,6 +616,8 @@ def main():
     ap.add_argument("--obj", type=str, default="", help="a relief mesh (OBJ, or .obj.xz) instead of the synthetic one")
     ap.add_argument("--freqs", type=str, default="", help="comma-separated band frequencies (Hz) instead of --bands")
     ap.add_argument("--workers", type=int, default=4, help="server workers (concurrent band solves)")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="hardware queues of the process (GPU_MAX_HW_QUEUES, set before the GPU is touched)")
     ap.add_argument("--sequential", action="store_true", help="solve the bands one after another (no server)")
     ap.add_argument("--gradient", action="store_true",
                     help="adjoint-gradient workload instead (forward + backward, stored vs recompute)")
@@ -715,7 +730,8 @@ def main():
                                                                                                      len(dirs)),
                       "solves_per_step": nsolve, "leaf": 32, "radius_factor": 1.5, "aca_tol": 1e-4,
                       "true_residual_tol": 1e-6, "c0": C0,
-                      "serving": "sequential bands" if args.sequential else "server, %d workers" % args.workers},
+                      "serving": "sequential bands" if args.sequential else "server, %d workers" % args.workers,
+                      "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
            "latency_ms": {"p50": 1000 * float(np.percentile(lat, 50)), "p99": 1000 * float(np.percentile(lat, 99)),
                           "note": ("a solve's latency is its band's wall time (37 rhs per batched solve)"
                                    if args.sequential else
