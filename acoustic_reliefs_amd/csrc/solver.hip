@@ -68,7 +68,10 @@ struct RhsState {
 constexpr int kVT = 256;      // threads per block of vector kernels
 constexpr double kInnerFloor = 1e-7;  // mixed refinement: tightest inner (FP32-operator) tolerance
 constexpr int kInnerBudget = 256;      // mixed refinement: iterations an inner solve may take
-constexpr int kMaxK = 8;      // iterations per host synchronisation, at most
+#ifndef ARH_SOLVER_MAXK
+#define ARH_SOLVER_MAXK 8
+#endif
+constexpr int kMaxK = ARH_SOLVER_MAXK;  // iterations per host synchronisation, at most
 constexpr int kMaxNB = 128;   // blocks per rhs of a reduction
 
 __device__ __forceinline__ void block_reduce4(double v[4], double *sh) {
