@@ -635,6 +635,7 @@ def main():
     ap.add_argument("--diffbem", action="store_true",
                     help="the drop-in DiffBEM at a reference optimiser's settings (value, gradient, CPU oracle band)")
     ap.add_argument("--leaf", type=int, default=128, help="with --diffbem: cluster_size")
+    ap.add_argument("--hleaf", type=int, default=32, help="H-matrix leaf size of the headline workload")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the isolated timing of the dominant kernel (the roofline figures)")
     ap.add_argument("--kernel-counts", action="store_true",
@@ -676,7 +677,7 @@ def main():
     P, E, dirs, L, bands = make_workload(args.grid, obj=args.obj)
     bands = [float(f) for f in args.freqs.split(",")] if args.freqs else bands[:args.bands]
     ctx = capi.Context(0)
-    g = capi.Geometry.from_mesh(ctx, P, E, leaf=32, factor=1.5)
+    g = capi.Geometry.from_mesh(ctx, P, E, leaf=args.hleaf, factor=1.5)
     nsolve = len(bands) * len(dirs)
     log("workload: N=%d, %d dense / %d low-rank blocks, %d bands x %d incidences" % (g.N, len(g.dense),
                                                                                     len(g.lowrank), len(bands),
@@ -686,7 +687,7 @@ def main():
             return run_mixed(g, bands, dirs, L, capi, lat)
     else:
         srv = capi.Server(workers=args.workers)
-        mid = srv.add_mesh(P, E, leaf=32, factor=1.5)
+        mid = srv.add_mesh(P, E, leaf=args.hleaf, factor=1.5)
 
         def step(lat):
             return run_served(srv, mid, bands, dirs, L, capi, lat)
@@ -728,7 +729,7 @@ def main():
                                    "0.6 m relief, %dx%d top (N=%d)" % (args.grid, args.grid, g.N)) +
                                   ", %d bands 250-4000 Hz x %d plane waves, 181 receivers at 5 m" % (len(bands),
                                                                                                      len(dirs)),
-                      "solves_per_step": nsolve, "leaf": 32, "radius_factor": 1.5, "aca_tol": 1e-4,
+                      "solves_per_step": nsolve, "leaf": args.hleaf, "radius_factor": 1.5, "aca_tol": 1e-4,
                       "true_residual_tol": 1e-6, "c0": C0,
                       "serving": "sequential bands" if args.sequential else "server, %d workers" % args.workers,
                       "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
