@@ -67,6 +67,10 @@ struct RhsState {
 
 constexpr int kVT = 256;      // threads per block of vector kernels
 constexpr double kInnerFloor = 1e-7;  // mixed refinement: tightest inner (FP32-operator) tolerance
+#ifndef ARH_INNER_FRAC
+#define ARH_INNER_FRAC 0.5
+#endif
+constexpr double kInnerFrac = ARH_INNER_FRAC;  // mixed refinement: inner target as a fraction of the remaining one
 constexpr int kInnerBudget = 256;      // mixed refinement: iterations an inner solve may take
 #ifndef ARH_SOLVER_MAXK
 #define ARH_SOLVER_MAXK 8
@@ -664,7 +668,7 @@ void solve(arh_hmat *h, int transpose, int R, const double2 *b, double2 *x, cons
             // right-hand side goes to FP64 after a bounded FP32 attempt
             std::vector<double> itol(R);
             for (int j = 0; j < R; j++)
-                itol[j] = std::min(0.5, std::max(kInnerFloor, 0.5 * tol / std::max(bnorm_rel[j], 1e-300)));
+                itol[j] = std::min(0.5, std::max(kInnerFloor, kInnerFrac * tol / std::max(bnorm_rel[j], 1e-300)));
             bicgstab(h, transpose, ARH_MIXED, R, rb.as<double2>(), dx.as<double2>(), redo, itol, maxit, o, st, iters,
                      err, W, NB, sweep == 0, stall, kInnerBudget);
             for (int j = 0; j < R; j++) iters_tot[j] += redo[j] ? iters[j] : 0;
