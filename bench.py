@@ -607,8 +607,8 @@ def run_stream(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)  # 5 steps of ~0.3 s: less run-to-run noise than one
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bands", type=int, default=13, help="number of bands (from 250 Hz up); 13 = all")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
