@@ -19,18 +19,20 @@ import os
 import sys
 import time
 
-# Hardware queues of this process (HIP's GPU_MAX_HW_QUEUES; 4 when unset). The served run's four worker
-# streams share the process with the main thread's context stream and the runtime's own stream: on 4 queues
-# two of the workers end up on one queue and take turns (two bands finish early, two late). 8 gives every
-# stream a queue of its own. The HIP runtime reads the variable when it starts, so it is set here, before
-# anything touches the GPU; --hw-queues chooses another value.
-HW_QUEUES = "8"
-for _i, _a in enumerate(sys.argv):
-    if _a == "--hw-queues" and _i + 1 < len(sys.argv):
-        HW_QUEUES = sys.argv[_i + 1]
-    elif _a.startswith("--hw-queues="):
-        HW_QUEUES = _a.split("=", 1)[1]
-os.environ["GPU_MAX_HW_QUEUES"] = HW_QUEUES
+
+def set_hw_queues(argv):
+    """Hardware queues of this process (HIP's GPU_MAX_HW_QUEUES; 4 when unset). The served run's four worker
+    streams share the process with the main thread's context stream and the runtime's own stream: on 4 queues
+    two of the workers end up on one queue and take turns (two bands finish early, two late). 8 gives every
+    stream a queue of its own. The HIP runtime reads the variable when it starts, so this runs before anything
+    touches the GPU; --hw-queues chooses another value."""
+    q = "8"
+    for i, a in enumerate(argv):
+        if a == "--hw-queues" and i + 1 < len(argv):
+            q = argv[i + 1]
+        elif a.startswith("--hw-queues="):
+            q = a.split("=", 1)[1]
+    os.environ["GPU_MAX_HW_QUEUES"] = q
 
 import numpy as np
 
@@ -802,4 +804,5 @@ def main():
 
 
 if __name__ == "__main__":
+    set_hw_queues(sys.argv)
     main()
